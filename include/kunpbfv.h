@@ -1,0 +1,222 @@
+/*
+ * kunpbfv -- C ABI of the MI355X-native BFV engine (libkunpbfv.so).
+ *
+ * Every entry point takes plain pointers, sizes and a hipStream_t passed as
+ * void*.  Device buffers are owned by the caller; the library owns only its
+ * context tables.  No entry point allocates device memory except context
+ * creation; multi-kernel operations take a caller workspace whose size is
+ * reported by the matching *_workspace_bytes() call.  Status: 0 on success,
+ * negative on error (kbfv_last_error() has the message).  No exception crosses
+ * this boundary.
+ *
+ * Data layout in device memory (uint64 residues, canonical in [0, q)):
+ *   polynomial over a basis     [limbs][N], limb i at +i*N
+ *   ciphertext                  [2][qn][N]   = (a | b), NTT domain, decrypts as b + a*s
+ *   batch of ciphertexts        [batch][2][qn][N]
+ *   secret key                  [pn+qn][N]   NTT domain over P then Q
+ *   public key                  [2][qn][N]   (a | b = e - a*s), NTT domain
+ *   relinearisation key         [2][dnum][ri_blocks][tn][N], NTT domain over T
+ *   plaintext (encoded)         [N] coefficients mod t
+ * Modulus indices (kbfv_context_info) run over one table [P | Q | R | T | t].
+ *
+ * Reference interfaces replaced (QiiPatri/kunpBFV, header-only CUDA C++; it has
+ * no C ABI, so each entry point names the C++ member or kernel launcher whose
+ * job it takes over):
+ */
+#ifndef KUNPBFV_H
+#define KUNPBFV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KBFV_OK 0
+#define KBFV_EINVAL (-1)
+#define KBFV_EHIP (-2)
+#define KBFV_ENOMEM (-3)
+#define KBFV_ESTATE (-4)
+
+typedef struct kbfv_context kbfv_context;
+
+/* Message of the last failed call on this thread. */
+const char *kbfv_last_error(void);
+
+/* ---------------------------------------------------------------- parameters */
+
+/* 60-bit chain: the largest primes below 2^60 that are 1 mod 2^17, taken in
+ * descending order as T (tn), then P (pn), then Q (qn).  With tn = 4 the T part
+ * equals the reference's tVec (include/Params.cuh:129-140). */
+int kbfv_chain_60bit(int qn, int pn, int tn, uint64_t *q, uint64_t *p, uint64_t *t);
+
+/* Primes below `upper` stepping by 2N (replaces get_primes_below,
+ * include/uintarith_bfv.h:983); used for the R basis. */
+int kbfv_primes_below(uint64_t ntt_size, uint64_t upper, int count, uint64_t *out);
+
+/* Default parameter set of a ring degree (replaces BFVContext::getPrimeBFV,
+ * include/Params.cuh:7-148): fills at most 64 entries of q/p/t. */
+int kbfv_default_params(int log_n, uint64_t *q, int *qn, uint64_t *p, int *pn, uint64_t *t,
+                        int *tn, int *gamma, uint64_t *plain);
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces BFVContext::BFVContext + preComputeOnCPU + copyMemoryToGPU
+ * (include/BFVcontext.cuh:13-101, 130-542, 545-1395).  Tables are per context
+ * (kernel arguments, no __constant__ globals), so contexts of different N
+ * coexist.  Fails with KBFV_EINVAL if the T basis cannot hold the key-switch
+ * inner product (T <= 4 * dnum * N * max Q_j * max R_b). */
+int kbfv_context_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, int pn,
+                        const uint64_t *t, int tn, int gamma, uint64_t plain, kbfv_context **out);
+int kbfv_context_destroy(kbfv_context *ctx);
+/* out[16] = log_n, n, qn, pn, tn, rn, gamma, dnum, ri_blocks, batching, nmod,
+ *           oP, oQ, oR, oT, oPlain */
+int kbfv_context_info(const kbfv_context *ctx, int *out);
+int kbfv_context_moduli(const kbfv_context *ctx, uint64_t *out);
+/* host copy of a named precomputed table (parity tests); returns count */
+int kbfv_context_export(const kbfv_context *ctx, const char *name, uint64_t *out, int cap);
+int kbfv_context_export_f64(const kbfv_context *ctx, const char *name, double *out, int cap);
+
+/* ---------------------------------------------------------------- NTT */
+
+/* Replaces BFVContext::ToNTTInplace / FromNTTInplace and their _for_QR,
+ * _for_externalProduct and _for_BFV twins (include/BFVcontext.cuh:1513-1677).
+ * dir = +1 forward, -1 inverse.  Element (b, y, l) lives at
+ *   src + b*s_batch + y*s_poly + l*s_limb  (in uint64 units; same for dst)
+ * and limb l uses modulus mod0 + l.  src may equal dst. */
+int kbfv_ntt(const kbfv_context *ctx, int dir, const uint64_t *src, uint64_t *dst, int mod0,
+             int limbs, int polys, int batch, int64_t s_batch, int64_t s_poly, int64_t s_limb,
+             int64_t d_batch, int64_t d_poly, int64_t d_limb, void *stream);
+
+/* ---------------------------------------------------------------- RNS kernels */
+
+/* element-wise over `polys` polynomials [polys][limbs][N] (replaces
+ * poly_add/sub/barrett batch kernels, include/poly_arithmetic.cuh:26-360) */
+int kbfv_poly_add(const kbfv_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
+                  int mod0, int limbs, int polys, void *stream);
+int kbfv_poly_sub(const kbfv_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
+                  int mod0, int limbs, int polys, void *stream);
+int kbfv_poly_mul(const kbfv_context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
+                  int mod0, int limbs, int polys, void *stream);
+
+/* HPS fast base conversion Q -> R / R -> Q, coefficient form
+ * (replaces BaseConverter::bConv_HPS, include/BasisConv_QR.cuh:191-208).
+ * in [polys][in_limbs][N] at in_stride elements per poly, out likewise. */
+int kbfv_bconv_q_to_r(const kbfv_context *ctx, const uint64_t *x, int64_t in_stride, uint64_t *y,
+                      int64_t out_stride, int polys, void *stream);
+int kbfv_bconv_r_to_q(const kbfv_context *ctx, const uint64_t *x, int64_t in_stride, uint64_t *y,
+                      int64_t out_stride, int polys, void *stream);
+
+/* NTT-domain 2x2 tensor over `limbs` limbs starting at mod0
+ * (replaces tensor_prod_2x2_rns_poly, include/poly_arithmetic.cuh:1180):
+ * ct1, ct2 [2][limbs][N] -> out [3][limbs][N] = (a1a2, a1b2+a2b1, b1b2). */
+int kbfv_tensor(const kbfv_context *ctx, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out,
+                int mod0, int limbs, int batch, void *stream);
+
+/* HPS scale-and-round t/Q over QR -> R (replaces scaleAndRound_HPS_QR_R,
+ * include/poly_arithmetic.cuh:1263-1317). x [polys][qn+rn][N] -> y [polys][rn][N]. */
+int kbfv_scale_round(const kbfv_context *ctx, const uint64_t *x, uint64_t *y, int polys,
+                     void *stream);
+
+/* HPS decryption scale-and-round (replaces hps_decrypt_scale_and_round,
+ * include/BFVcontext.cuh:1680-1717): x [batch][qn][N] coefficient form of
+ * b + a*s -> m [batch][N]. */
+int kbfv_hps_decrypt(const kbfv_context *ctx, const uint64_t *x, uint64_t *m, int batch,
+                     void *stream);
+
+/* ---------------------------------------------------------------- key switching */
+
+/* Replaces BFVContext::modUpQjtoT_23 (include/BasisConv.cuh:137):
+ * x [batch][qn][N] coeff -> out [batch][dnum][tn][N] coeff. */
+int kbfv_modup_qj_to_t(const kbfv_context *ctx, const uint64_t *x, uint64_t *out, int batch,
+                       void *stream);
+/* Replaces BFVContext::modUpPQtoT_23 (include/BasisConv.cuh:61):
+ * x [batch][pn+qn][N] -> out [batch][ri_blocks][tn][N]. */
+int kbfv_modup_pq_to_t(const kbfv_context *ctx, const uint64_t *x, uint64_t *out, int batch,
+                       void *stream);
+/* Replaces BFVContext::external_product_T / _swk_reuse
+ * (include/ExternalProduct.cuh:85-177): key-stationary inner product in T,
+ * cin [batch][dnum][tn][N] (NTT) x key [2][dnum][ri_blocks][tn][N] (NTT)
+ * -> out [batch][2][ri_blocks][tn][N].  The key is read once per call. */
+int kbfv_inner_product(const kbfv_context *ctx, const uint64_t *cin, const uint64_t *key,
+                       uint64_t *out, int batch, void *stream);
+/* Replaces BFVContext::modUpTtoPQl_23 (include/BasisConv.cuh:215):
+ * in [batch][2][ri_blocks][tn][N] coeff -> out [batch][2][pn+qn][N]. */
+int kbfv_modup_t_to_pq(const kbfv_context *ctx, const uint64_t *in, uint64_t *out, int batch,
+                       void *stream);
+/* Replaces BFVContext::modDownPQltoQl_23 (include/BasisConv.cuh:300):
+ * in [batch][2][pn+qn][N] -> out [batch][2][qn][N]; when add != NULL the
+ * [batch][2][qn][N] polynomials `add` (at add_stride per batch entry) are added. */
+int kbfv_moddown(const kbfv_context *ctx, const uint64_t *in, uint64_t *out, const uint64_t *add,
+                 int64_t add_stride, int batch, void *stream);
+
+/* ---------------------------------------------------------------- randomness */
+
+/* XSalsa20 keystream (replaces RNG::generateRandom_device, include/RNG.cuh:18) */
+int kbfv_xsalsa20(const uint8_t key[32], const uint8_t nonce[24], uint64_t block0, uint8_t *out,
+                  size_t len, void *stream);
+/* Samplers (replace Sampler::*, include/Sampler_impl.cuh:13-107).  kind:
+ * 0 uniform mod q, 1 ternary, 2 zero-one (1/2, 1/4, 1/4), 3 CBD k=21.
+ * Stream of (seed, tag); out [limbs][N] for moduli mod0.. */
+int kbfv_sample(const kbfv_context *ctx, int kind, const uint8_t seed[32], uint64_t tag, int mod0,
+                int limbs, uint64_t *out, void *stream);
+
+/* ---------------------------------------------------------------- BFV scheme */
+
+/* SecretKey ctor (include/SecretKey.cuh:20-37): ternary s over PQ, NTT form */
+int kbfv_keygen_sk(const kbfv_context *ctx, const uint8_t seed[32], uint64_t *sk, void *stream);
+/* BFVScheme::addEncKey (include/BFVScheme.cuh:57-85) */
+int kbfv_keygen_pk(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *sk,
+                   uint64_t *pk, void *stream);
+/* BFVScheme::addMultKey_23 (include/KeySwitch_23.cuh:21-66) */
+size_t kbfv_keygen_rlk_workspace_bytes(const kbfv_context *ctx);
+int kbfv_keygen_rlk(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *sk,
+                    uint64_t *rlk, void *workspace, size_t ws_bytes, void *stream);
+/* BFVScheme::encryptMsg (include/BFVScheme.cuh:118-146); entry b of the batch
+ * draws its randomness with tag index b.  m [batch][N] encoded plaintexts. */
+size_t kbfv_encrypt_workspace_bytes(const kbfv_context *ctx, int batch);
+int kbfv_encrypt(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *pk,
+                 const uint64_t *m, uint64_t *ct, int batch, void *workspace, size_t ws_bytes,
+                 void *stream);
+/* BFVScheme::decryptMsg (include/BFVScheme.cuh:148-166) -> m [batch][N] */
+size_t kbfv_decrypt_workspace_bytes(const kbfv_context *ctx, int batch);
+int kbfv_decrypt(const kbfv_context *ctx, const uint64_t *sk, const uint64_t *ct, uint64_t *m,
+                 int batch, void *workspace, size_t ws_bytes, void *stream);
+/* BFVScheme::add / addAndEqual (include/BFVScheme.cuh:169-190) */
+int kbfv_add(const kbfv_context *ctx, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out,
+             int batch, void *stream);
+
+/* HomMult + relinearisation (replaces BFVScheme::multAndEqual_23 / mult_23,
+ * include/KeySwitch_23.cuh:192-310, and compute_t_QInv_c0c1c2 :703-926).
+ * Batched form: every kernel covers the whole batch and the relinearisation key
+ * is read once per call.  ct1, ct2, out: [batch][2][qn][N]; out may alias ct1. */
+size_t kbfv_mult_workspace_bytes(const kbfv_context *ctx, int batch);
+int kbfv_mult_relin(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t *ct1,
+                    const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
+                    size_t ws_bytes, void *stream);
+
+/* Batch encoder (BFVContext::encode / decode, include/Encoder.cuh:23-113):
+ * INTT / NTT over t when t = 1 mod 2N, identity (coefficient encoding) otherwise.
+ * In place on [batch][N]. */
+int kbfv_encode(const kbfv_context *ctx, uint64_t *vals, int batch, void *stream);
+int kbfv_decode(const kbfv_context *ctx, uint64_t *vals, int batch, void *stream);
+
+/* ---------------------------------------------------------------- RGSW x RLWE */
+
+/* No counterpart in the reference (its "external product" is the key-switch
+ * inner product).  Gadget: base 2^base_bits digits of each Q residue. */
+int kbfv_rgsw_rows(const kbfv_context *ctx, int base_bits);
+size_t kbfv_rgsw_encrypt_workspace_bytes(const kbfv_context *ctx);
+int kbfv_rgsw_encrypt(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *sk,
+                      const uint64_t *m_coeff, int base_bits, uint64_t *rgsw, void *workspace,
+                      size_t ws_bytes, void *stream);
+size_t kbfv_external_product_workspace_bytes(const kbfv_context *ctx, int batch, int base_bits);
+int kbfv_external_product(const kbfv_context *ctx, const uint64_t *rgsw, const uint64_t *ct,
+                          uint64_t *out, int batch, int base_bits, void *workspace,
+                          size_t ws_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

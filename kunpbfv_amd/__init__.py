@@ -1,0 +1,384 @@
+"""kunpbfv_amd -- MI355X-native BFV engine, Python host binding.
+
+Loads the in-tree ``libkunpbfv.so`` (HIP for gfx950) through its C ABI
+(``include/kunpbfv.h``) and mirrors the reference's BFVContext / BFVScheme
+operations on device buffers.  Device memory and streams come from PyTorch
+(``torch.int64`` tensors carry the uint64 residues bit for bit); every compute
+call goes to the HIP library -- there is no CPU fallback, and a missing or
+unloadable library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkunpbfv.so")
+
+_vp, _i64, _u64, _sz = C.c_void_p, C.c_int64, C.c_uint64, C.c_size_t
+_u64p = C.POINTER(C.c_uint64)
+_f64p = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); the complete exported C ABI
+SIGNATURES = {
+    "kbfv_last_error": (C.c_char_p, []),
+    "kbfv_chain_60bit": (C.c_int, [C.c_int, C.c_int, C.c_int, _u64p, _u64p, _u64p]),
+    "kbfv_primes_below": (C.c_int, [_u64, _u64, C.c_int, _u64p]),
+    "kbfv_default_params": (C.c_int, [C.c_int, _u64p, _ip, _u64p, _ip, _u64p, _ip, _ip, _u64p]),
+    "kbfv_context_create": (C.c_int, [C.c_int, _u64p, C.c_int, _u64p, C.c_int, _u64p, C.c_int, C.c_int,
+                                      _u64, C.POINTER(_vp)]),
+    "kbfv_context_destroy": (C.c_int, [_vp]),
+    "kbfv_context_info": (C.c_int, [_vp, _ip]),
+    "kbfv_context_moduli": (C.c_int, [_vp, _u64p]),
+    "kbfv_context_export": (C.c_int, [_vp, C.c_char_p, _u64p, C.c_int]),
+    "kbfv_context_export_f64": (C.c_int, [_vp, C.c_char_p, _f64p, C.c_int]),
+    "kbfv_ntt": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _i64, _i64, _i64,
+                           _i64, _i64, _i64, _vp]),
+    "kbfv_poly_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "kbfv_poly_sub": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "kbfv_poly_mul": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "kbfv_bconv_q_to_r": (C.c_int, [_vp, _vp, _i64, _vp, _i64, C.c_int, _vp]),
+    "kbfv_bconv_r_to_q": (C.c_int, [_vp, _vp, _i64, _vp, _i64, C.c_int, _vp]),
+    "kbfv_tensor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "kbfv_scale_round": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_hps_decrypt": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_modup_qj_to_t": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_modup_pq_to_t": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_inner_product": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_modup_t_to_pq": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_moddown": (C.c_int, [_vp, _vp, _vp, _vp, _i64, C.c_int, _vp]),
+    "kbfv_xsalsa20": (C.c_int, [_vp, _vp, _u64, _vp, _sz, _vp]),
+    "kbfv_sample": (C.c_int, [_vp, C.c_int, _vp, _u64, C.c_int, C.c_int, _vp, _vp]),
+    "kbfv_keygen_sk": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "kbfv_keygen_pk": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "kbfv_keygen_rlk_workspace_bytes": (_sz, [_vp]),
+    "kbfv_keygen_rlk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "kbfv_encrypt_workspace_bytes": (_sz, [_vp, C.c_int]),
+    "kbfv_encrypt": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_decrypt_workspace_bytes": (_sz, [_vp, C.c_int]),
+    "kbfv_decrypt": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
+    "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_encode": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "kbfv_decode": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "kbfv_rgsw_rows": (C.c_int, [_vp, C.c_int]),
+    "kbfv_rgsw_encrypt_workspace_bytes": (_sz, [_vp]),
+    "kbfv_rgsw_encrypt": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _sz, _vp]),
+    "kbfv_external_product_workspace_bytes": (_sz, [_vp, C.c_int, C.c_int]),
+    "kbfv_external_product": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _sz, _vp]),
+}
+
+
+class KbfvError(RuntimeError):
+    pass
+
+
+def _load(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise KbfvError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != 0:
+        raise KbfvError(f"kunpbfv error {rc}: {lib.kbfv_last_error().decode()}")
+    return rc
+
+
+def _arr(vals):
+    a = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64))
+    return a, a.ctypes.data_as(_u64p)
+
+
+def chain_60bit(qn, pn, tn):
+    q, p, t = (np.zeros(n, np.uint64) for n in (qn, pn, tn))
+    check(lib.kbfv_chain_60bit(qn, pn, tn, q.ctypes.data_as(_u64p), p.ctypes.data_as(_u64p),
+                               t.ctypes.data_as(_u64p)))
+    return [int(x) for x in q], [int(x) for x in p], [int(x) for x in t]
+
+
+def primes_below(ntt_size, upper, count):
+    out = np.zeros(count, np.uint64)
+    check(lib.kbfv_primes_below(ntt_size, upper, count, out.ctypes.data_as(_u64p)))
+    return [int(x) for x in out]
+
+
+def default_params(log_n):
+    q, p, t = (np.zeros(64, np.uint64) for _ in range(3))
+    qn, pn, tn, gamma = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    plain = C.c_uint64()
+    check(lib.kbfv_default_params(log_n, q.ctypes.data_as(_u64p), C.byref(qn), p.ctypes.data_as(_u64p),
+                                  C.byref(pn), t.ctypes.data_as(_u64p), C.byref(tn), C.byref(gamma),
+                                  C.byref(plain)))
+    return dict(log_n=log_n, q=[int(x) for x in q[:qn.value]], p=[int(x) for x in p[:pn.value]],
+                t=[int(x) for x in t[:tn.value]], gamma=gamma.value, plain=plain.value)
+
+
+def _torch():
+    import torch  # device memory and streams only
+    return torch
+
+
+def ptr(t):
+    """device pointer of a contiguous torch tensor"""
+    assert t.is_contiguous(), "kunpbfv needs contiguous device buffers"
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def to_device(a, device="cuda"):
+    """numpy uint64 -> torch.int64 device tensor (same bits)"""
+    torch = _torch()
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+    return torch.from_numpy(a.view(np.int64)).to(device)
+
+
+def to_host(t):
+    """torch.int64 device tensor -> numpy uint64 (same bits)"""
+    return t.detach().cpu().numpy().view(np.uint64)
+
+
+def _seed(seed):
+    b = bytes(seed)
+    assert len(b) == 32, "seeds are 32 bytes"
+    return C.create_string_buffer(b, 32)
+
+
+class Context:
+    """A BFV parameter set with its device tables (replaces BFVContext)."""
+
+    INFO = ("log_n", "n", "qn", "pn", "tn", "rn", "gamma", "dnum", "ri_blocks", "batching", "nmod",
+            "oP", "oQ", "oR", "oT", "oPlain")
+
+    def __init__(self, log_n, q, p, t, gamma, plain):
+        self._q, qp = _arr(q)
+        self._p, pp = _arr(p)
+        self._t, tp = _arr(t)
+        h = C.c_void_p()
+        check(lib.kbfv_context_create(log_n, qp, len(q), pp, len(p), tp, len(t), gamma, plain,
+                                      C.byref(h)))
+        self.h = h
+        info = (C.c_int * 16)()
+        check(lib.kbfv_context_info(self.h, info))
+        for k, v in zip(self.INFO, info):
+            setattr(self, k, v)
+        self.plain = int(plain)
+        mods = np.zeros(self.nmod, np.uint64)
+        lib.kbfv_context_moduli(self.h, mods.ctypes.data_as(_u64p))
+        self.mod = mods
+        self._ws = None
+
+    @classmethod
+    def default(cls, log_n):
+        d = default_params(log_n)
+        return cls(d["log_n"], d["q"], d["p"], d["t"], d["gamma"], d["plain"])
+
+    @classmethod
+    def chain60(cls, log_n, qn, pn=1, tn=4, gamma=2, plain=65537):
+        q, p, t = chain_60bit(qn, pn, tn)
+        return cls(log_n, q, p, t, gamma, plain)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.kbfv_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ tables
+    def export(self, name):
+        out = np.zeros(1 << 16, np.uint64)
+        n = lib.kbfv_context_export(self.h, name.encode(), out.ctypes.data_as(_u64p), len(out))
+        check(0 if n > 0 else n)
+        return out[:n].copy()
+
+    def export_f64(self, name):
+        out = np.zeros(4096, np.float64)
+        n = lib.kbfv_context_export_f64(self.h, name.encode(), out.ctypes.data_as(_f64p), len(out))
+        check(0 if n > 0 else n)
+        return out[:n].copy()
+
+    # ------------------------------------------------------------ buffers
+    def zeros(self, *shape):
+        torch = _torch()
+        return torch.zeros(shape, dtype=torch.int64, device="cuda")
+
+    def empty(self, *shape):
+        torch = _torch()
+        return torch.empty(shape, dtype=torch.int64, device="cuda")
+
+    def workspace(self, nbytes):
+        """grow-only scratch buffer owned by this context object"""
+        torch = _torch()
+        nbytes = max(int(nbytes), 256)
+        if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        return self._ws
+
+    # ------------------------------------------------------------ transforms
+    def ntt(self, src, dst, direction, mod0, limbs, polys=1, batch=1, s=None, d=None, stream=None):
+        N = self.n
+        s = s or (polys * limbs * N, limbs * N, N)
+        d = d or s
+        check(lib.kbfv_ntt(self.h, direction, ptr(src), ptr(dst), mod0, limbs, polys, batch, *s, *d,
+                           stream_ptr(stream)))
+        return dst
+
+    def poly_op(self, op, a, b, out, mod0, limbs, polys=1, stream=None):
+        fn = {"add": lib.kbfv_poly_add, "sub": lib.kbfv_poly_sub, "mul": lib.kbfv_poly_mul}[op]
+        check(fn(self.h, ptr(a), ptr(b), ptr(out), mod0, limbs, polys, stream_ptr(stream)))
+        return out
+
+    def bconv(self, x, y, q_to_r=True, polys=1, stream=None):
+        fn = lib.kbfv_bconv_q_to_r if q_to_r else lib.kbfv_bconv_r_to_q
+        ins, outs = (self.qn, self.rn) if q_to_r else (self.rn, self.qn)
+        check(fn(self.h, ptr(x), ins * self.n, ptr(y), outs * self.n, polys, stream_ptr(stream)))
+        return y
+
+    def tensor(self, c1, c2, out, mod0, limbs, batch=1, stream=None):
+        check(lib.kbfv_tensor(self.h, ptr(c1), ptr(c2), ptr(out), mod0, limbs, batch, stream_ptr(stream)))
+        return out
+
+    def scale_round(self, x, y, polys=1, stream=None):
+        check(lib.kbfv_scale_round(self.h, ptr(x), ptr(y), polys, stream_ptr(stream)))
+        return y
+
+    def hps_decrypt(self, x, m, batch=1, stream=None):
+        check(lib.kbfv_hps_decrypt(self.h, ptr(x), ptr(m), batch, stream_ptr(stream)))
+        return m
+
+    def modup_qj_to_t(self, x, out, batch=1, stream=None):
+        check(lib.kbfv_modup_qj_to_t(self.h, ptr(x), ptr(out), batch, stream_ptr(stream)))
+        return out
+
+    def modup_pq_to_t(self, x, out, batch=1, stream=None):
+        check(lib.kbfv_modup_pq_to_t(self.h, ptr(x), ptr(out), batch, stream_ptr(stream)))
+        return out
+
+    def inner_product(self, cin, key, out, batch=1, stream=None):
+        check(lib.kbfv_inner_product(self.h, ptr(cin), ptr(key), ptr(out), batch, stream_ptr(stream)))
+        return out
+
+    def modup_t_to_pq(self, x, out, batch=1, stream=None):
+        check(lib.kbfv_modup_t_to_pq(self.h, ptr(x), ptr(out), batch, stream_ptr(stream)))
+        return out
+
+    def moddown(self, x, out, add=None, add_stride=0, batch=1, stream=None):
+        a = ptr(add) if add is not None else None
+        check(lib.kbfv_moddown(self.h, ptr(x), ptr(out), a, add_stride, batch, stream_ptr(stream)))
+        return out
+
+    def sample(self, kind, seed, tag, mod0, limbs, out, stream=None):
+        kinds = {"uniform": 0, "ternary": 1, "zo": 2, "cbd": 3}
+        sb = _seed(seed)
+        check(lib.kbfv_sample(self.h, kinds[kind], sb, tag, mod0, limbs, ptr(out), stream_ptr(stream)))
+        return out
+
+    # ------------------------------------------------------------ scheme
+    def keygen_sk(self, seed, stream=None):
+        sk = self.empty(self.pn + self.qn, self.n)
+        check(lib.kbfv_keygen_sk(self.h, _seed(seed), ptr(sk), stream_ptr(stream)))
+        return sk
+
+    def keygen_pk(self, seed, sk, stream=None):
+        pk = self.empty(2, self.qn, self.n)
+        check(lib.kbfv_keygen_pk(self.h, _seed(seed), ptr(sk), ptr(pk), stream_ptr(stream)))
+        return pk
+
+    def keygen_rlk(self, seed, sk, stream=None):
+        rlk = self.empty(2, self.dnum, self.ri_blocks, self.tn, self.n)
+        nb = lib.kbfv_keygen_rlk_workspace_bytes(self.h)
+        ws = self.workspace(nb)
+        check(lib.kbfv_keygen_rlk(self.h, _seed(seed), ptr(sk), ptr(rlk), ptr(ws), ws.numel(),
+                                  stream_ptr(stream)))
+        return rlk
+
+    def encrypt(self, seed, pk, m, stream=None):
+        """m: [batch][N] encoded plaintexts -> [batch][2][qn][N]"""
+        batch = m.shape[0]
+        ct = self.empty(batch, 2, self.qn, self.n)
+        ws = self.workspace(lib.kbfv_encrypt_workspace_bytes(self.h, batch))
+        check(lib.kbfv_encrypt(self.h, _seed(seed), ptr(pk), ptr(m), ptr(ct), batch, ptr(ws), ws.numel(),
+                               stream_ptr(stream)))
+        return ct
+
+    def decrypt(self, sk, ct, stream=None):
+        batch = ct.shape[0]
+        m = self.empty(batch, self.n)
+        ws = self.workspace(lib.kbfv_decrypt_workspace_bytes(self.h, batch))
+        check(lib.kbfv_decrypt(self.h, ptr(sk), ptr(ct), ptr(m), batch, ptr(ws), ws.numel(),
+                               stream_ptr(stream)))
+        return m
+
+    def add(self, c1, c2, out=None, stream=None):
+        out = out if out is not None else self.empty(*c1.shape)
+        check(lib.kbfv_add(self.h, ptr(c1), ptr(c2), ptr(out), c1.shape[0], stream_ptr(stream)))
+        return out
+
+    def mult_workspace_bytes(self, batch):
+        return lib.kbfv_mult_workspace_bytes(self.h, batch)
+
+    def mult_relin(self, rlk, c1, c2, out=None, ws=None, stream=None):
+        batch = c1.shape[0]
+        out = out if out is not None else self.empty(*c1.shape)
+        ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
+        check(lib.kbfv_mult_relin(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
+                                  stream_ptr(stream)))
+        return out
+
+    def encode(self, vals, stream=None):
+        check(lib.kbfv_encode(self.h, ptr(vals), vals.shape[0], stream_ptr(stream)))
+        return vals
+
+    def decode(self, vals, stream=None):
+        check(lib.kbfv_decode(self.h, ptr(vals), vals.shape[0], stream_ptr(stream)))
+        return vals
+
+    def rgsw_rows(self, base_bits):
+        r = lib.kbfv_rgsw_rows(self.h, base_bits)
+        check(0 if r > 0 else r)
+        return r
+
+    def rgsw_encrypt(self, seed, sk, m_coeff, base_bits, stream=None):
+        rows = self.rgsw_rows(base_bits)
+        out = self.empty(rows, 2, self.qn, self.n)
+        ws = self.workspace(lib.kbfv_rgsw_encrypt_workspace_bytes(self.h))
+        check(lib.kbfv_rgsw_encrypt(self.h, _seed(seed), ptr(sk), ptr(m_coeff), base_bits, ptr(out), ptr(ws),
+                                    ws.numel(), stream_ptr(stream)))
+        return out
+
+    def external_product(self, rgsw, ct, base_bits, out=None, stream=None):
+        batch = ct.shape[0]
+        out = out if out is not None else self.empty(*ct.shape)
+        ws = self.workspace(lib.kbfv_external_product_workspace_bytes(self.h, batch, base_bits))
+        check(lib.kbfv_external_product(self.h, ptr(rgsw), ptr(ct), ptr(out), batch, base_bits, ptr(ws),
+                                        ws.numel(), stream_ptr(stream)))
+        return out
+
+
+def xsalsa20(key, nonce, length, block0=0, stream=None):
+    torch = _torch()
+    out = torch.empty(length, dtype=torch.uint8, device="cuda")
+    kb = C.create_string_buffer(bytes(key), 32)
+    nb = C.create_string_buffer(bytes(nonce), 24)
+    check(lib.kbfv_xsalsa20(kb, nb, block0, ptr(out), length, stream_ptr(stream)))
+    return bytes(out.cpu().numpy())

@@ -1,0 +1,118 @@
+// Internal host-side definitions shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kbfv_device.h"
+#include "kunpbfv.h"
+
+typedef unsigned __int128 kb_u128_host;
+
+// ---------------------------------------------------------------- errors
+void kb_set_error(const char *fmt, ...);
+#define KB_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            kb_set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call, hipGetErrorString(_e)); \
+            return KBFV_EHIP;                                                              \
+        }                                                                                  \
+    } while (0)
+#define KB_REQUIRE(cond, ...)           \
+    do {                                \
+        if (!(cond)) {                  \
+            kb_set_error(__VA_ARGS__);  \
+            return KBFV_EINVAL;         \
+        }                               \
+    } while (0)
+#define KB_LAUNCH_CHECK()                                                                    \
+    do {                                                                                     \
+        hipError_t _e = hipGetLastError();                                                   \
+        if (_e != hipSuccess) {                                                              \
+            kb_set_error("%s:%d kernel launch -> %s", __FILE__, __LINE__, hipGetErrorString(_e)); \
+            return KBFV_EHIP;                                                                \
+        }                                                                                    \
+    } while (0)
+
+// ---------------------------------------------------------------- context
+// Device pointers of every precomputed table.  Small tables live in one
+// device allocation (blob); twiddles in their own.
+struct kb_dev_tables {
+    const uint64_t *mod;   // [nmod]
+    const uint64_t *mu;    // [nmod][2] Barrett floor(2^128/q): lo, hi
+    const kbfv_tw *tw;     // [nmod][2][N]: forward psi table, inverse psi table (bit-reversed)
+    const kbfv_tw *ninv;   // [nmod] n^-1 with Shoup companion
+
+    // HPS base conversion Q -> R and R -> Q
+    const kbfv_tw *q_hatinv;       // [qn]
+    const uint64_t *q_hat_mod_r;   // [rn][qn]
+    const uint64_t *alphaQ_mod_r;  // [qn+1][rn]
+    const double *q_inv;           // [qn]
+    const kbfv_tw *r_hatinv;       // [rn]
+    const uint64_t *r_hat_mod_q;   // [qn][rn]
+    const uint64_t *alphaR_mod_q;  // [rn+1][qn]
+    const double *r_inv;           // [rn]
+    // HPS scale-and-round QR -> R
+    const uint64_t *sr_modr;  // [rn][qn+1]
+    const double *sr_frac;    // [qn]
+    // HPS decrypt
+    const kbfv_tw *dec_modt;    // [qn] (value, Shoup mod t)
+    const double *dec_frac;     // [qn]
+    const kbfv_tw *dec_B_modt;  // [qn]
+    const double *dec_B_frac;   // [qn]
+    // encrypt
+    const kbfv_tw *tinv_mod_q;  // [qn]
+    // key switching (full level)
+    const kbfv_tw *ri_hatinv;        // [ri_blocks][gamma]
+    const uint64_t *ri_hat_mod_t;    // [ri_blocks][tn][gamma]
+    const uint64_t *ri_mod_t;        // [tn][ri_blocks]
+    const kbfv_tw *qj_hatinv;        // [dnum][pn]
+    const uint64_t *qj_hat_mod_t;    // [dnum][tn][pn]
+    const uint64_t *qj_mod_t;        // [dnum][tn]
+    const kbfv_tw *t_hatinv;         // [tn]
+    const uint64_t *t_hat_mod_pq;    // [pn+qn][tn]
+    const uint64_t *t_mod_pq;        // [pn+qn]
+    const uint64_t *halfT_mod;       // [pn+qn+tn] in the order P, Q, T
+    const uint64_t *p_hat_mod_q;     // [qn][pn]
+    const kbfv_tw *p_hatinv;         // [pn]
+    const kbfv_tw *Pinv_mod_q;       // [qn]
+    const uint64_t *P_mod_q;         // [qn]
+};
+
+struct kbfv_context {
+    int log_n = 0, n = 0;
+    int qn = 0, pn = 0, tn = 0, rn = 0, gamma = 0, dnum = 0, ri_blocks = 0;
+    int batching = 0, nmod = 0, oP = 0, oQ = 0, oR = 0, oT = 0, oPlain = 0;
+    uint64_t plain = 0;
+    int qMSB = 0, sizeQMSB = 0, tMSB = 0;
+    uint64_t negQ_mod_t = 0, negQ_mod_t_shoup = 0;
+    int device = 0;
+
+    // host copies (exported to parity tests)
+    std::vector<uint64_t> mod, psi;
+    std::vector<std::pair<std::string, std::vector<uint64_t>>> host_u64;
+    std::vector<std::pair<std::string, std::vector<double>>> host_f64;
+
+    void *d_blob = nullptr;
+    kbfv_tw *d_tw = nullptr;
+    kb_dev_tables t{};
+};
+
+// per-limb modulus helpers
+static inline uint64_t kb_host_mulmod(uint64_t a, uint64_t b, uint64_t q) {
+    return (uint64_t)(((kb_u128_host)a * b) % q);
+}
+
+// ---------------------------------------------------------------- launchers (ntt.hip)
+// dir: +1 forward, -1 inverse.  Layout addressed as
+//   src + bi*sB + y*sY + l*sL   (elements),   dst + bi*dB + y*dY + l*dL
+// for batch bi < batch, poly y < polys, limb l < limbs; limb l uses modulus mod0 + l.
+int kb_ntt(const kbfv_context *c, int dir, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
+           int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
+           hipStream_t st);
+// contiguous [batch*polys][limbs][N] in place
+int kb_ntt_inplace(const kbfv_context *c, int dir, uint64_t *data, int mod0, int limbs, int polys,
+                   hipStream_t st);

@@ -1,0 +1,314 @@
+// Hybrid key switching in the auxiliary basis T (the reference's "23" method).
+//
+// Replaces modUpPQtoT_23_kernel (include/BasisConv.cuh:11-59),
+// modUpQjtoT_23_kernel (:86-135), external_product_T_kernel and its
+// _swk_reuse variant (include/ExternalProduct.cuh:11-177), modUpTtoPQl_23_kernel
+// (include/BasisConv.cuh:159-213) and the modDownPQltoQl_23 kernels (:233-298).
+// The float overflow estimates (`vv += (float)x / (float)m`, truncation, and
+// the 128-bit wrap of `acc - vv * (M mod m)`) are kept operation for operation;
+// the surrounding arithmetic is exact with canonical outputs.
+//
+// MI355X design: with one special prime (K = 1) the digit decomposition of a
+// limb is the limb itself, so "modUp Qj -> T then NTT over T" is one NTT that
+// reads the Q limb directly (kb_digits_to_t_ntt).  The inner product is
+// key-stationary: each thread holds its key column (2 x dnum values) in
+// registers and walks the whole batch, so the relinearisation key is read from
+// HBM once per call instead of once per ciphertext.
+#include "kbfv_internal.h"
+#include "kbfv_ops.h"
+
+namespace {
+
+constexpr int KS_BS = 256;
+constexpr int KS_MAXG = 16;  // max gamma / tn held in registers
+
+inline unsigned blocks_for(long total, int bs) { return (unsigned)((total + bs - 1) / bs); }
+
+// x: [batch][qn][N] -> out: [batch][dnum][tn][N]
+__global__ void k_modup_qj_to_t(kb_dev_tables t, const uint64_t *__restrict__ x, long xs,
+                                uint64_t *__restrict__ out, int pn, int qn, int tn, int dnum, int oQ,
+                                int oT, int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = (int)(i % n);
+    long rest = i / n;
+    const int b = (int)(rest % dnum);
+    const long bt = rest / dnum;
+    const uint64_t *xp = x + bt * xs + k;
+    uint64_t rb[KS_MAXG];
+    float vv = 0;
+    int cnt = 0;
+    for (int j = 0; j < pn && b * pn + j < qn; ++j, ++cnt) {
+        const int qi = b * pn + j;
+        const uint64_t q = t.mod[oQ + qi];
+        const kbfv_tw h = t.qj_hatinv[b * pn + j];
+        rb[j] = kb_shoup(xp[(long)qi * n], h.w, h.ws, q);
+        const float rr = (float)rb[j];
+        vv += rr / (float)q;
+    }
+    const int vf = (int)vv;
+    uint64_t *op = out + (bt * dnum + b) * (long)tn * n + k;
+    for (int kk = 0; kk < tn; ++kk) {
+        const int m = oT + kk;
+        kbfv_u128 acc{0, 0};
+        for (int j = 0; j < cnt; ++j) kb_mac128(acc, rb[j], t.qj_hat_mod_t[(b * tn + kk) * pn + j]);
+        kb_sub128(acc, (uint64_t)vf * t.qj_mod_t[b * tn + kk]);
+        op[(long)kk * n] = kb_barrett128(acc, t.mod[m], t.mu[2 * m], t.mu[2 * m + 1]);
+    }
+}
+
+// x: [batch][pn+qn][N] -> out: [batch][ri_blocks][tn][N]
+__global__ void k_modup_pq_to_t(kb_dev_tables t, const uint64_t *__restrict__ x,
+                                uint64_t *__restrict__ out, int pq, int tn, int gamma, int B, int oT,
+                                int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = (int)(i % n);
+    long rest = i / n;
+    const int b = (int)(rest % B);
+    const long bt = rest / B;
+    const uint64_t *xp = x + bt * (long)pq * n + k;
+    uint64_t rb[KS_MAXG];
+    float vv = 0;
+    int cnt = 0;
+    for (int j = 0; j < gamma && b * gamma + j < pq; ++j, ++cnt) {
+        const int idx = b * gamma + j;
+        const uint64_t r = t.mod[idx];  // P then Q are the first moduli
+        const kbfv_tw h = t.ri_hatinv[idx];
+        rb[j] = kb_shoup(xp[(long)idx * n], h.w, h.ws, r);
+        vv += (float)rb[j] / (float)r;
+    }
+    const int vf = (int)vv;
+    uint64_t *op = out + (bt * B + b) * (long)tn * n + k;
+    for (int kk = 0; kk < tn; ++kk) {
+        const int m = oT + kk;
+        kbfv_u128 acc{0, 0};
+        for (int j = 0; j < cnt; ++j) kb_mac128(acc, rb[j], t.ri_hat_mod_t[(b * tn + kk) * gamma + j]);
+        kb_sub128(acc, (uint64_t)vf * t.ri_mod_t[kk * B + b]);
+        op[(long)kk * n] = kb_barrett128(acc, t.mod[m], t.mu[2 * m], t.mu[2 * m + 1]);
+    }
+}
+
+// Key-stationary inner product.  cin [batch][D][tn][N], key [2][D][B][tn][N],
+// out [batch][2][B][tn][N].  Thread = (coefficient, T limb k, block b).
+template <int MAXD>
+__global__ void __launch_bounds__(KS_BS)
+    k_inner_product(kb_dev_tables t, const uint64_t *__restrict__ cin, const uint64_t *__restrict__ key,
+                    uint64_t *__restrict__ out, int D, int B, int tn, int oT, int n, int batch) {
+    // block index -> (b fastest, then coefficient tile, then T limb): the B
+    // workgroups that share one slice of cin run side by side.
+    const int tiles = n / KS_BS;
+    int bid = blockIdx.x;
+    const int b = bid % B;
+    bid /= B;
+    const int tile = bid % tiles;
+    const int kk = bid / tiles;
+    const int k = tile * KS_BS + threadIdx.x;
+    const int m = oT + kk;
+    const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+    const long N = n;
+    uint64_t ka[MAXD], kbv[MAXD];
+    const long kstride = (long)B * tn * N;  // digit stride in the key
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i) {
+        if (i < D) {
+            const long off = (long)i * kstride + ((long)b * tn + kk) * N + k;
+            ka[i] = key[off];
+            kbv[i] = key[(long)D * kstride + off];
+        }
+    }
+    for (int bt = 0; bt < batch; ++bt) {
+        const uint64_t *cp = cin + ((long)bt * D * tn + kk) * N + k;
+        kbfv_u128 a0{0, 0}, a1{0, 0};
+#pragma unroll
+        for (int i = 0; i < MAXD; ++i) {
+            if (i < D) {
+                const uint64_t cv = cp[(long)i * tn * N];
+                kb_mac128(a0, cv, ka[i]);
+                kb_mac128(a1, cv, kbv[i]);
+            }
+        }
+        uint64_t *op = out + (long)bt * 2 * B * tn * N + ((long)b * tn + kk) * N + k;
+        op[0] = kb_barrett128(a0, q, mlo, mhi);
+        op[(long)B * tn * N] = kb_barrett128(a1, q, mlo, mhi);
+    }
+}
+
+// fallback for dnum > 64: one thread per output, key re-read per ciphertext
+__global__ void k_inner_product_generic(kb_dev_tables t, const uint64_t *__restrict__ cin,
+                                        const uint64_t *__restrict__ key, uint64_t *__restrict__ out,
+                                        int D, int B, int tn, int oT, int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long N = n;
+    const int k = (int)(i % n);
+    long rest = i / n;
+    const int kk = (int)(rest % tn);
+    rest /= tn;
+    const int b = (int)(rest % B);
+    rest /= B;
+    const int h = (int)(rest % 2);
+    const long bt = rest / 2;
+    const int m = oT + kk;
+    kbfv_u128 acc{0, 0};
+    for (int d = 0; d < D; ++d)
+        kb_mac128(acc, cin[((bt * D + d) * tn + kk) * N + k],
+                  key[(((long)h * D + d) * B + b) * tn * N + (long)kk * N + k]);
+    out[i] = kb_barrett128(acc, t.mod[m], t.mu[2 * m], t.mu[2 * m + 1]);
+}
+
+// in [batch][2][B][tn][N] coefficient form -> out [batch][2][pq][N]
+__global__ void k_modup_t_to_pq(kb_dev_tables t, const uint64_t *__restrict__ in,
+                                uint64_t *__restrict__ out, int pq, int tn, int gamma, int B, int oT,
+                                int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = (int)(i % n);
+    long rest = i / n;
+    const int b = (int)(rest % B);
+    rest /= B;  // rest = bt*2 + h
+    const uint64_t *ip = in + (rest * B + b) * (long)tn * n + k;
+    uint64_t rb[KS_MAXG];
+    float vv = 0;
+    for (int kk = 0; kk < tn; ++kk) {
+        const uint64_t tm = t.mod[oT + kk];
+        const uint64_t ra = ip[(long)kk * n] + t.halfT_mod[pq + kk];
+        const kbfv_tw h = t.t_hatinv[kk];
+        rb[kk] = kb_shoup(ra, h.w, h.ws, tm);
+        const float rr = (float)rb[kk];
+        vv += rr / (float)tm;
+    }
+    const int vf = (int)vv;
+    uint64_t *op = out + rest * (long)pq * n + k;
+    for (int j = 0; j < gamma && b * gamma + j < pq; ++j) {
+        const int idx = b * gamma + j;
+        const uint64_t r = t.mod[idx];
+        kbfv_u128 acc{0, 0};
+        for (int kk = 0; kk < tn; ++kk) kb_mac128(acc, rb[kk], t.t_hat_mod_pq[idx * tn + kk]);
+        kb_sub128(acc, (uint64_t)vf * t.t_mod_pq[idx]);
+        uint64_t v = kb_barrett128(acc, r, t.mu[2 * idx], t.mu[2 * idx + 1]);
+        v = kb_csub(v + r - t.halfT_mod[idx], r);
+        op[(long)idx * n] = v;
+    }
+}
+
+// in [batch][2][pq][N] -> out [batch][2][qn][N] (+ add [batch](2 polys at add_stride))
+__global__ void k_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                          const uint64_t *__restrict__ add, long add_stride, int pn, int qn, int oQ,
+                          int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = (int)(i % n);
+    long rest = i / n;
+    const int l = (int)(rest % qn);
+    rest /= qn;  // bt*2 + h
+    const int h = (int)(rest % 2);
+    const long bt = rest / 2;
+    const int pq = pn + qn;
+    const uint64_t *ip = in + rest * (long)pq * n + k;
+    const int m = oQ + l;
+    const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+    uint64_t ra;
+    if (pn == 1) {
+        ra = kb_barrett128(kbfv_u128{ip[0], 0}, q, mlo, mhi);
+    } else {
+        kbfv_u128 acc{0, 0};
+        for (int j = 0; j < pn; ++j) {
+            const kbfv_tw hi = t.p_hatinv[j];
+            const uint64_t rb = kb_shoup(ip[(long)j * n], hi.w, hi.ws, t.mod[j]);
+            kb_mac128(acc, rb, t.p_hat_mod_q[l * pn + j]);
+        }
+        ra = kb_barrett128(acc, q, mlo, mhi);
+    }
+    ra = q - ra + ip[(long)(pn + l) * n];
+    const kbfv_tw pi = t.Pinv_mod_q[l];
+    uint64_t v = kb_shoup(ra, pi.w, pi.ws, q);
+    if (add) v = kb_addmod(v, add[bt * add_stride + ((long)h * qn + l) * n + k], q);
+    out[i] = v;
+}
+
+}  // namespace
+
+int kb_modup_qj_to_t(const kbfv_context *c, const uint64_t *x, long xs, uint64_t *out, int batch,
+                     hipStream_t st) {
+    KB_REQUIRE(c && x && out && batch > 0, "bad modUp arguments");
+    KB_REQUIRE(c->pn <= KS_MAXG, "too many special primes");
+    const long total = (long)batch * c->dnum * c->n;
+    hipLaunchKernelGGL(k_modup_qj_to_t, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t, x, xs, out,
+                       c->pn, c->qn, c->tn, c->dnum, c->oQ, c->oT, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int kb_modup_pq_to_t(const kbfv_context *c, const uint64_t *x, uint64_t *out, int batch, hipStream_t st) {
+    KB_REQUIRE(c && x && out && batch > 0, "bad modUp arguments");
+    KB_REQUIRE(c->gamma <= KS_MAXG, "gamma too large");
+    const long total = (long)batch * c->ri_blocks * c->n;
+    hipLaunchKernelGGL(k_modup_pq_to_t, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t, x, out,
+                       c->pn + c->qn, c->tn, c->gamma, c->ri_blocks, c->oT, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t *key, uint64_t *out,
+                     int batch, hipStream_t st) {
+    KB_REQUIRE(c && cin && key && out && batch > 0, "bad inner-product arguments");
+    const int D = c->dnum, B = c->ri_blocks, tn = c->tn, n = c->n;
+    const unsigned grid = (unsigned)((n / KS_BS) * tn * B);
+#define KB_IP(MD)                                                                                    \
+    hipLaunchKernelGGL(k_inner_product<MD>, dim3(grid), dim3(KS_BS), 0, st, c->t, cin, key, out, D, B, \
+                       tn, c->oT, n, batch)
+    if (D <= 4)
+        KB_IP(4);
+    else if (D <= 8)
+        KB_IP(8);
+    else if (D <= 16)
+        KB_IP(16);
+    else if (D <= 32)
+        KB_IP(32);
+    else {
+        const long total = (long)batch * 2 * B * tn * n;
+        hipLaunchKernelGGL(k_inner_product_generic, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t,
+                           cin, key, out, D, B, tn, c->oT, n, total);
+    }
+#undef KB_IP
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int kb_modup_t_to_pq(const kbfv_context *c, const uint64_t *in, uint64_t *out, int batch, hipStream_t st) {
+    KB_REQUIRE(c && in && out && batch > 0, "bad modUp arguments");
+    KB_REQUIRE(c->tn <= KS_MAXG, "too many T primes");
+    const long total = (long)batch * 2 * c->ri_blocks * c->n;
+    hipLaunchKernelGGL(k_modup_t_to_pq, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t, in, out,
+                       c->pn + c->qn, c->tn, c->gamma, c->ri_blocks, c->oT, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int kb_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
+               long add_stride, int batch, hipStream_t st) {
+    KB_REQUIRE(c && in && out && batch > 0, "bad modDown arguments");
+    const long total = (long)batch * 2 * c->qn * c->n;
+    hipLaunchKernelGGL(k_moddown, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t, in, out, add,
+                       add_stride, c->pn, c->qn, c->oQ, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int kb_digits_to_t_ntt(const kbfv_context *c, const uint64_t *x_coeff, long x_stride, uint64_t *out,
+                       int batch, uint64_t *tmp, hipStream_t st) {
+    const long N = c->n;
+    const int D = c->dnum, tn = c->tn;
+    if (c->pn == 1) {
+        // digit d is Q limb d itself: NTT over each T prime straight from the Q limb
+        // (x < q_d < 2^60 is a valid lazy input for every T prime)
+        return kb_ntt(c, +1, x_coeff, out, c->oT, tn, D, batch, x_stride, N, 0, (long)D * tn * N,
+                      (long)tn * N, N, st);
+    }
+    (void)tmp;
+    int rc = kb_modup_qj_to_t(c, x_coeff, x_stride, out, batch, st);
+    if (rc) return rc;
+    return kb_ntt(c, +1, out, out, c->oT, tn, D, batch, (long)D * tn * N, (long)tn * N, N,
+                  (long)D * tn * N, (long)tn * N, N, st);
+}

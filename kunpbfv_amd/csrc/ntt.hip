@@ -1,0 +1,280 @@
+// Batched negacyclic NTT / INTT over 60-bit primes for gfx950.
+//
+// Computes the same transform as the reference's two-kernel radix-8 NTT
+// (include/ntt_60bit.cuh:414-823, dispatch include/BFVcontext.cuh:1513-1552):
+// Cooley-Tukey forward with the bit-reversed psi table (stage with m groups uses
+// psi[m + group]), Gentleman-Sande inverse with the bit-reversed psi^-1 table
+// whose entry 1 carries n^-1, outputs canonical in [0, q).
+//
+// Design (MI355X): N = N1 * N2 four-step split into two passes, each a kernel
+// that stages a 32 KiB tile in LDS and runs radix-16 rounds in registers:
+//   column pass  -- C adjacent columns of stride N/N1 (coalesced C*8-byte rows),
+//                   global stages 0 .. log2(N1)-1
+//   row pass     -- C contiguous chunks of N2 elements, remaining stages
+// Forward = column then row; inverse = row then column.  Values stay lazy in
+// [0, 4q) (forward) / [0, 2q) (inverse) between stages and are reduced once on
+// the final store.  Twiddles are (w, Shoup w') pairs read as one 16-byte load,
+// one load per distinct twiddle per thread.  The grid is ordered limb-slowest so
+// that every workgroup in flight shares one modulus' tables in L2.
+#include "kbfv_internal.h"
+
+namespace {
+
+constexpr int LOG_R = 4;  // radix-16 register rounds
+constexpr int TILE = 4096;  // elements per workgroup
+
+template <int LOG_S, bool COLUMN>
+struct Geo {
+    static constexpr int S = 1 << LOG_S;
+    static constexpr int C = TILE / S;         // sub-transforms per workgroup
+    static constexpr int T = TILE >> LOG_R;    // threads per workgroup (256)
+    static constexpr int ROWP = S + S / 16;    // row pass: one pad word per 16
+    static constexpr int WORDS = COLUMN ? TILE : C * ROWP;
+    __device__ static __forceinline__ int addr(int c, int k) {
+        if constexpr (COLUMN)
+            return k * C + c;
+        else
+            return c * ROWP + k + (k >> 4);
+    }
+};
+
+__device__ __forceinline__ void ct_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t q,
+                                        uint64_t q2) {
+    a = kb_csub(a, q2);                          // [0, 2q)
+    uint64_t t = kb_shoup_lazy(b, w.w, w.ws, q); // [0, 2q)
+    b = a + q2 - t;
+    a = a + t;
+}
+
+__device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t q,
+                                        uint64_t q2) {
+    uint64_t u = kb_csub(a + b, q2);
+    uint64_t v = kb_shoup_lazy(a + q2 - b, w.w, w.ws, q);
+    a = u;
+    b = v;
+}
+
+// One register round of LR levels starting at pass level lam0.
+template <int LR, int LOG_S, bool COLUMN, bool FWD, bool NINV>
+__device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const kbfv_tw *tab,
+                                          const int log_n, const int g0, const int grp0,
+                                          const int stride, const uint64_t q, const kbfv_tw ninv) {
+    using G = Geo<LOG_S, COLUMN>;
+    constexpr int S = G::S, C = G::C, RR = 1 << LR;
+    constexpr int NG = C * (S >> LR);
+    const uint64_t q2 = q << 1;
+    const int Bs = S >> lam0;
+    const int d = Bs >> LR;
+    for (int grp = threadIdx.x; grp < NG; grp += G::T) {
+        int c, u;
+        if constexpr (COLUMN) {
+            c = grp % C;
+            u = grp / C;
+        } else {
+            u = grp % (S >> LR);
+            c = grp / (S >> LR);
+        }
+        const int kbase = (u / d) * Bs + (u % d);
+        // global position of element k of sub-transform c
+        const int posc = COLUMN ? (grp0 * C + c) : (grp0 * C + c) * S;
+        const int pstep = COLUMN ? stride : 1;
+        uint64_t x[RR];
+#pragma unroll
+        for (int i = 0; i < RR; ++i) x[i] = tile[G::addr(c, kbase + i * d)];
+        if constexpr (FWD) {
+#pragma unroll
+            for (int j = 0; j < LR; ++j) {
+                const int h = RR >> (j + 1);
+                const int gst = g0 + lam0 + j;
+#pragma unroll
+                for (int m = 0; m < (1 << j); ++m) {
+                    const int i0 = m * (2 * h);
+                    const int pos = posc + (kbase + i0 * d) * pstep;
+                    const kbfv_tw w = tab[(1 << gst) + (pos >> (log_n - gst))];
+#pragma unroll
+                    for (int i = i0; i < i0 + h; ++i) ct_bfly(x[i], x[i + h], w, q, q2);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = LR - 1; j >= 0; --j) {
+                const int h = RR >> (j + 1);
+                const int gst = g0 + lam0 + j;
+#pragma unroll
+                for (int m = 0; m < (1 << j); ++m) {
+                    const int i0 = m * (2 * h);
+                    const int pos = posc + (kbase + i0 * d) * pstep;
+                    const kbfv_tw w = tab[(1 << gst) + (pos >> (log_n - gst))];
+#pragma unroll
+                    for (int i = i0; i < i0 + h; ++i) {
+                        gs_bfly(x[i], x[i + h], w, q, q2);
+                        if (NINV && gst == 0) x[i] = kb_shoup_lazy(x[i], ninv.w, ninv.ws, q);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RR; ++i) tile[G::addr(c, kbase + i * d)] = x[i];
+    }
+    __syncthreads();
+}
+
+template <int LOG_S, bool COLUMN, bool FWD>
+__global__ void __launch_bounds__(TILE >> LOG_R)
+    k_ntt_pass(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+               const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
+               const uint64_t *__restrict__ mods, int log_n, int mod0, int limbs, int polys,
+               int batch, int groups, long sB, long sY, long sL, long dB, long dY, long dL) {
+    using G = Geo<LOG_S, COLUMN>;
+    constexpr int S = G::S, C = G::C, T = G::T;
+    constexpr bool LAST = (FWD != COLUMN);  // forward ends with the row pass
+    __shared__ uint64_t tile[G::WORDS];
+
+    int bid = blockIdx.x;
+    const int grp0 = bid % groups;
+    bid /= groups;
+    const int y = bid % polys;
+    bid /= polys;
+    const int bi = bid % batch;
+    const int l = bid / batch;
+
+    const int n = 1 << log_n;
+    const int m = mod0 + l;
+    const uint64_t q = mods[m];
+    const kbfv_tw *tab = tw + ((size_t)m * 2 + (FWD ? 0 : 1)) * n;
+    const uint64_t *in = src + bi * sB + y * sY + l * sL;
+    uint64_t *out = dst + bi * dB + y * dY + l * dL;
+    const int stride = n >> LOG_S;
+    const int g0 = COLUMN ? 0 : (log_n - LOG_S);
+
+#pragma unroll
+    for (int j = 0; j < TILE / T; ++j) {
+        const int e = threadIdx.x + j * T;
+        int c, k;
+        if constexpr (COLUMN) {
+            c = e % C;
+            k = e / C;
+        } else {
+            c = e / S;
+            k = e % S;
+        }
+        const int pos = COLUMN ? (grp0 * C + c + k * stride) : ((grp0 * C + c) * S + k);
+        tile[G::addr(c, k)] = in[pos];
+    }
+    __syncthreads();
+
+    constexpr int LR0 = LOG_S < LOG_R ? LOG_S : LOG_R;
+    constexpr int REM1 = LOG_S - LR0;
+    constexpr int LR1 = REM1 < LOG_R ? REM1 : LOG_R;
+    constexpr int LR2 = REM1 - LR1;
+    static_assert(LR2 <= LOG_R, "at most three rounds");
+    const kbfv_tw ninv = (!FWD && LAST) ? ninv_tab[m] : kbfv_tw{0, 0};
+    constexpr bool NI = (!FWD && LAST);
+    if constexpr (FWD) {
+        ntt_round<LR0, LOG_S, COLUMN, true, false>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv);
+        if constexpr (LR1 > 0)
+            ntt_round<LR1, LOG_S, COLUMN, true, false>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
+        if constexpr (LR2 > 0)
+            ntt_round<LR2, LOG_S, COLUMN, true, false>(tile, LR0 + LR1, tab, log_n, g0, grp0, stride, q,
+                                                       ninv);
+    } else {
+        if constexpr (LR2 > 0)
+            ntt_round<LR2, LOG_S, COLUMN, false, NI>(tile, LR0 + LR1, tab, log_n, g0, grp0, stride, q,
+                                                     ninv);
+        if constexpr (LR1 > 0)
+            ntt_round<LR1, LOG_S, COLUMN, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
+        ntt_round<LR0, LOG_S, COLUMN, false, NI>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv);
+    }
+
+    const uint64_t q2 = q << 1;
+#pragma unroll
+    for (int j = 0; j < TILE / T; ++j) {
+        const int e = threadIdx.x + j * T;
+        int c, k;
+        if constexpr (COLUMN) {
+            c = e % C;
+            k = e / C;
+        } else {
+            c = e / S;
+            k = e % S;
+        }
+        const int pos = COLUMN ? (grp0 * C + c + k * stride) : ((grp0 * C + c) * S + k);
+        uint64_t v = tile[G::addr(c, k)];
+        if constexpr (LAST) {
+            if constexpr (FWD) v = kb_csub(v, q2);
+            v = kb_csub(v, q);
+        }
+        out[pos] = v;
+    }
+}
+
+template <int LOG_S, bool COLUMN, bool FWD>
+int launch_pass(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
+                int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
+                hipStream_t st) {
+    using G = Geo<LOG_S, COLUMN>;
+    const int subs = c->n >> LOG_S;  // columns (column pass) or chunks (row pass)
+    KB_REQUIRE(subs >= G::C, "ring degree 2^%d too small for the NTT tiling", c->log_n);
+    const int groups = subs / G::C;
+    const long blocks = (long)groups * limbs * polys * batch;
+    KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
+    hipLaunchKernelGGL((k_ntt_pass<LOG_S, COLUMN, FWD>), dim3((unsigned)blocks), dim3(G::T), 0, st,
+                       src, dst, c->t.tw, c->t.ninv, c->t.mod, c->log_n, mod0, limbs, polys, batch,
+                       groups, sB, sY, sL, dB, dY, dL);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+template <bool COLUMN, bool FWD>
+int launch_pass_s(int log_s, const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0,
+                  int limbs, int polys, int batch, long sB, long sY, long sL, long dB, long dY,
+                  long dL, hipStream_t st) {
+    switch (log_s) {
+        case 6:
+            return launch_pass<6, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB,
+                                               dY, dL, st);
+        case 7:
+            return launch_pass<7, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB,
+                                               dY, dL, st);
+        case 8:
+            return launch_pass<8, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB,
+                                               dY, dL, st);
+        default:
+            kb_set_error("unsupported NTT pass size 2^%d", log_s);
+            return KBFV_EINVAL;
+    }
+}
+
+}  // namespace
+
+int kb_ntt(const kbfv_context *c, int dir, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
+           int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
+           hipStream_t st) {
+    KB_REQUIRE(c && src && dst, "null argument");
+    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
+    KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod, "modulus range out of bounds");
+    KB_REQUIRE(polys > 0 && batch > 0, "empty NTT");
+    if (mod0 + limbs - 1 == c->oPlain) KB_REQUIRE(c->batching, "plain modulus has no 2N-th root");
+    // column pass takes ceil(log_n / 2) levels, row pass the rest
+    const int lc = (c->log_n + 1) / 2, lr = c->log_n - lc;
+    int rc;
+    if (dir > 0) {
+        rc = launch_pass_s<true, true>(lc, c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY,
+                                       dL, st);
+        if (rc) return rc;
+        return launch_pass_s<false, true>(lr, c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB,
+                                          dY, dL, st);
+    }
+    rc = launch_pass_s<false, false>(lr, c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY,
+                                     dL, st);
+    if (rc) return rc;
+    return launch_pass_s<true, false>(lc, c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY,
+                                      dL, st);
+}
+
+int kb_ntt_inplace(const kbfv_context *c, int dir, uint64_t *data, int mod0, int limbs, int polys,
+                   hipStream_t st) {
+    const long N = c->n;
+    return kb_ntt(c, dir, data, data, mod0, limbs, polys, 1, 0, (long)limbs * N, N, 0,
+                  (long)limbs * N, N, st);
+}

@@ -1,0 +1,561 @@
+// BFV scheme pipelines and the extern "C" entry points of libkunpbfv.
+//
+// Replaces BFVScheme (include/BFVScheme.cuh:7-166), the relinearisation-key
+// generation and HomMult of include/KeySwitch_23.cuh (:21-66, :192-310,
+// :703-926) and the encoder (include/Encoder.cuh:23-113).
+//
+// HomMult + relinearisation, per batch chunk (every launch covers the chunk):
+//   INTT_Q(ct1, ct2)  -> bConv Q->R -> NTT_R  (Q part stays the NTT-domain input)
+//   tensor over QR    -> INTT_QR -> scale-and-round t/Q -> bConv R->Q
+//   a1a2: digits -> NTT_T (fused) -> key-stationary inner product -> INTT_T
+//         -> T->PQ -> modDown (+ a1b2+a2b1, b1b2) -> NTT_Q -> out
+// No device allocation happens here: intermediates live in the caller's
+// workspace (sizes from *_workspace_bytes), so a captured graph can replay it.
+#include <algorithm>
+#include <cstring>
+
+#include "kbfv_internal.h"
+#include "kbfv_ops.h"
+
+namespace {
+
+constexpr int BS = 256;
+inline unsigned blocks_for(long total, int bs) { return (unsigned)((total + bs - 1) / bs); }
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+// sampler tags: (purpose << 32) | index, the same layout the oracle uses
+constexpr uint64_t TAG(uint64_t purpose) { return purpose << 32; }
+enum : uint64_t {
+    TAG_SK = 1,
+    TAG_PK_A = 2,
+    TAG_PK_E = 3,
+    TAG_RLK_A = 4,
+    TAG_RLK_E = 5,
+    TAG_ENC_V = 6,
+    TAG_ENC_E0 = 7,
+    TAG_ENC_E1 = 8,
+    TAG_RGSW_A = 9,
+    TAG_RGSW_E = 10
+};
+
+// bump allocator over the caller's workspace (256-byte aligned slices); with a
+// null base it only measures.
+struct Bump {
+    uint8_t *base;
+    size_t used = 0;
+    explicit Bump(void *b) : base((uint8_t *)b) {}
+    uint64_t *take(size_t elems) {
+        used = (used + 255) & ~size_t(255);
+        uint64_t *p = base ? (uint64_t *)(base + used) : nullptr;
+        used += elems * sizeof(uint64_t);
+        return p;
+    }
+};
+
+// ---------------------------------------------------------------- small kernels
+// out = x + sign * a*s over [polys][limbs][N]; s: [limbs][N] broadcast over polys
+template <bool SUB>
+__global__ void k_mac_bcast(kb_dev_tables t, const uint64_t *__restrict__ x, const uint64_t *__restrict__ a,
+                            const uint64_t *__restrict__ s, uint64_t *__restrict__ out, int mod0, int limbs,
+                            int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long L = (long)limbs * n;
+    const long r = i % L;
+    const int m = mod0 + (int)(r / n);
+    const uint64_t q = t.mod[m];
+    const uint64_t p = kb_mulmod(a[i], s[r], q, t.mu[2 * m], t.mu[2 * m + 1]);
+    out[i] = SUB ? kb_submod(x[i], p, q) : kb_addmod(x[i], p, q);
+}
+
+// ct.a = v*pk.a + e0, ct.b = v*pk.b + u1   (v, e0, u1: [batch][qn][N]; pk broadcast)
+__global__ void k_enc_combine(kb_dev_tables t, const uint64_t *__restrict__ v,
+                              const uint64_t *__restrict__ e0, const uint64_t *__restrict__ u1,
+                              const uint64_t *__restrict__ pk, uint64_t *__restrict__ ct, int qn, int oQ,
+                              int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long L = (long)qn * n;
+    const long bt = i / L, r = i % L;
+    const int m = oQ + (int)(r / n);
+    const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+    const uint64_t vv = v[i];
+    ct[bt * 2 * L + r] = kb_addmod(kb_mulmod(vv, pk[r], q, mlo, mhi), e0[i], q);
+    ct[bt * 2 * L + L + r] = kb_addmod(kb_mulmod(vv, pk[L + r], q, mlo, mhi), u1[i], q);
+}
+
+// x = b + a*s over Q (ct [batch][2][qn][N], s_q [qn][N] broadcast)
+__global__ void k_dec_combine(kb_dev_tables t, const uint64_t *__restrict__ ct,
+                              const uint64_t *__restrict__ s_q, uint64_t *__restrict__ x, int qn, int oQ,
+                              int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long L = (long)qn * n;
+    const long bt = i / L, r = i % L;
+    const int m = oQ + (int)(r / n);
+    const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+    x[i] = kb_addmod(ct[bt * 2 * L + L + r], kb_mulmod(ct[bt * 2 * L + r], s_q[r], q, mlo, mhi), q);
+}
+
+// relinearisation key body for digit d: b = e + P*s^2*[limbs of digit d] - a*s  (over PQ)
+__global__ void k_rlk_combine(kb_dev_tables t, const uint64_t *__restrict__ e,
+                              const uint64_t *__restrict__ a, const uint64_t *__restrict__ s_pq,
+                              const uint64_t *__restrict__ s2_q, uint64_t *__restrict__ b, int pn, int qn,
+                              int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int pq = pn + qn;
+    const long L = (long)pq * n;
+    const long d = i / L, r = i % L;
+    const int l = (int)(r / n);  // modulus index (P then Q are the first moduli)
+    const uint64_t q = t.mod[l], mlo = t.mu[2 * l], mhi = t.mu[2 * l + 1];
+    uint64_t v = e[i];
+    const int ql = l - pn;
+    if (ql >= 0 && ql >= (int)d * pn && ql < (int)d * pn + pn) {
+        const uint64_t ps2 = kb_mulmod(s2_q[(long)ql * n + (r % n)], t.P_mod_q[ql], q, mlo, mhi);
+        v = kb_addmod(v, ps2, q);
+    }
+    b[i] = kb_submod(v, kb_mulmod(a[i], s_pq[r], q, mlo, mhi), q);
+}
+
+// signed lift of a plaintext polynomial m [N] (values above t/2 negative) into Q
+__global__ void k_lift_plain(kb_dev_tables t, const uint64_t *__restrict__ m, uint64_t *__restrict__ out,
+                             int oQ, uint64_t plain, int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int k = (int)(i % n), l = (int)(i / n);
+    const uint64_t q = t.mod[oQ + l];
+    const uint64_t v = m[k];
+    out[i] = v > plain / 2 ? q - (plain - v) : v;
+}
+
+// rgsw [rows][2][qn][N]: component 1 = e - component 0 * s
+__global__ void k_rgsw_body(kb_dev_tables t, const uint64_t *__restrict__ e, const uint64_t *__restrict__ s_q,
+                            uint64_t *__restrict__ rgsw, int qn, int oQ, int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long L = (long)qn * n;
+    const long r = i / L, off = i % L;
+    const int m = oQ + (int)(off / n);
+    const uint64_t q = t.mod[m];
+    const uint64_t a = rgsw[r * 2 * L + off];
+    rgsw[r * 2 * L + L + off] = kb_submod(e[i], kb_mulmod(a, s_q[off], q, t.mu[2 * m], t.mu[2 * m + 1]), q);
+}
+
+__global__ void k_fill_zero(uint64_t *__restrict__ p, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) p[i] = 0;
+}
+
+int mac_bcast(const kbfv_context *c, bool sub, const uint64_t *x, const uint64_t *a, const uint64_t *s,
+              uint64_t *out, int mod0, int limbs, int polys, hipStream_t st) {
+    const long total = (long)polys * limbs * c->n;
+    if (sub)
+        hipLaunchKernelGGL(k_mac_bcast<true>, dim3(blocks_for(total, BS)), dim3(BS), 0, st, c->t, x, a, s, out,
+                           mod0, limbs, c->n, total);
+    else
+        hipLaunchKernelGGL(k_mac_bcast<false>, dim3(blocks_for(total, BS)), dim3(BS), 0, st, c->t, x, a, s, out,
+                           mod0, limbs, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int fill_zero(uint64_t *p, long total, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(total, BS)), dim3(BS), 0, st, p, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+#define KB_TRY(x)                  \
+    do {                           \
+        int _rc = (x);             \
+        if (_rc) return _rc;       \
+    } while (0)
+
+// ---------------------------------------------------------------- HomMult layout
+struct MultWs {
+    uint64_t *X1, *X2;      // coefficient-form inputs [cb][2][qn][N]      (region A)
+    uint64_t *SR;           // scale-and-round out   [cb][3][rn][N]        (region A, later)
+    uint64_t *R1, *R2;      // R-extension           [cb][2][rn][N]        (region B)
+    uint64_t *DQ;           // t/Q-scaled tensor     [cb][3][qn][N]        (region B, later)
+    uint64_t *D3;           // QR tensor             [cb][3][qn+rn][N]     (region C)
+    uint64_t *CT;           // digits over T         [cb][dnum][tn][N]     (region C, later)
+    uint64_t *MD;           // modDown out           [cb][2][qn][N]        (region C, later)
+    uint64_t *IP;           // inner product         [cb][2][B][tn][N]     (region E)
+    uint64_t *PQ;           // T -> PQ               [cb][2][pq][N]        (region F)
+};
+
+size_t mult_layout(const kbfv_context *c, int cb, void *base, MultWs *w) {
+    const size_t N = c->n, qn = c->qn, rn = c->rn, pq = c->pn + c->qn;
+    const size_t A = std::max(4 * qn, 3 * rn) * N;
+    const size_t Bq = std::max(4 * rn, 3 * qn) * N;
+    const size_t Cc = std::max(std::max(3 * (qn + rn), (size_t)c->dnum * c->tn), 2 * qn) * N;
+    const size_t E = 2 * (size_t)c->ri_blocks * c->tn * N;
+    const size_t F = 2 * pq * N;
+    Bump b(base);
+    uint64_t *a = b.take(A * cb), *bb = b.take(Bq * cb), *cc = b.take(Cc * cb);
+    uint64_t *e = b.take(E * cb), *f = b.take(F * cb);
+    if (w) {
+        w->X1 = a;
+        w->X2 = a ? a + 2 * qn * N * cb : nullptr;
+        w->SR = a;
+        w->R1 = bb;
+        w->R2 = bb ? bb + 2 * rn * N * cb : nullptr;
+        w->DQ = bb;
+        w->D3 = cc;
+        w->CT = cc;
+        w->MD = cc;
+        w->IP = e;
+        w->PQ = f;
+    }
+    return b.used;
+}
+
+int mult_chunk(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1, const uint64_t *ct2,
+               uint64_t *out, int cb, const MultWs &w, hipStream_t st) {
+    const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn, pq = c->pn + c->qn;
+    const long CTS = 2 * qn * N;  // ciphertext stride
+    // 1. INTT of both inputs (out of place; the NTT-domain inputs stay for the tensor)
+    KB_TRY(kb_ntt(c, -1, ct1, w.X1, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
+    KB_TRY(kb_ntt(c, -1, ct2, w.X2, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
+    // 2. HPS base extension Q -> R, 3. NTT over R
+    KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 2 * cb, st));
+    KB_TRY(kb_bconv(c, true, w.X2, qn * N, w.R2, rn * N, 2 * cb, st));
+    KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * cb, 1, 0, rn * N, N, 0, rn * N, N, st));
+    // 4. tensor over QR, 5. INTT over QR
+    KB_TRY(kb_tensor_qr(c, ct1, ct2, w.R1, w.R2, w.D3, cb, st));
+    KB_TRY(kb_ntt(c, -1, w.D3, w.D3, c->oQ, sn, 3 * cb, 1, 0, sn * N, N, 0, sn * N, N, st));
+    // 6. scale by t/Q and round into R, 7. back to Q
+    KB_TRY(kb_scale_round(c, w.D3, w.SR, 3 * cb, st));
+    KB_TRY(kb_bconv(c, false, w.SR, rn * N, w.DQ, qn * N, 3 * cb, st));
+    // 8. relinearise a1a2: digits -> NTT over T
+    KB_TRY(kb_digits_to_t_ntt(c, w.DQ, 3 * qn * N, w.CT, cb, nullptr, st));
+    // 9. key-stationary inner product, 10. INTT over T
+    KB_TRY(kb_inner_product(c, w.CT, rlk, w.IP, cb, st));
+    KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * cb, 1, 0, (long)c->tn * N, N, 0,
+                  (long)c->tn * N, N, st));
+    // 11. T -> PQ, 12. modDown with the a1b2+a2b1 and b1b2 terms added
+    KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, cb, st));
+    KB_TRY(kb_moddown(c, w.PQ, w.MD, w.DQ + qn * N, 3 * qn * N, cb, st));
+    // 13. back to the NTT domain, into the output ciphertexts
+    (void)pq;
+    return kb_ntt(c, +1, w.MD, out, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- thin wrappers
+extern "C" int kbfv_ntt(const kbfv_context *c, int dir, const uint64_t *src, uint64_t *dst, int mod0,
+                        int limbs, int polys, int batch, int64_t sb, int64_t sp, int64_t sl, int64_t db,
+                        int64_t dp, int64_t dl, void *st) {
+    KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
+    return kb_ntt(c, dir, src, dst, mod0, limbs, polys, batch, sb, sp, sl, db, dp, dl, S(st));
+}
+extern "C" int kbfv_poly_add(const kbfv_context *c, const uint64_t *a, const uint64_t *b, uint64_t *o,
+                             int mod0, int limbs, int polys, void *st) {
+    return kb_poly_binop(c, 0, a, b, o, mod0, limbs, polys, S(st));
+}
+extern "C" int kbfv_poly_sub(const kbfv_context *c, const uint64_t *a, const uint64_t *b, uint64_t *o,
+                             int mod0, int limbs, int polys, void *st) {
+    return kb_poly_binop(c, 1, a, b, o, mod0, limbs, polys, S(st));
+}
+extern "C" int kbfv_poly_mul(const kbfv_context *c, const uint64_t *a, const uint64_t *b, uint64_t *o,
+                             int mod0, int limbs, int polys, void *st) {
+    return kb_poly_binop(c, 2, a, b, o, mod0, limbs, polys, S(st));
+}
+extern "C" int kbfv_bconv_q_to_r(const kbfv_context *c, const uint64_t *x, int64_t is, uint64_t *y,
+                                 int64_t os, int polys, void *st) {
+    return kb_bconv(c, true, x, is, y, os, polys, S(st));
+}
+extern "C" int kbfv_bconv_r_to_q(const kbfv_context *c, const uint64_t *x, int64_t is, uint64_t *y,
+                                 int64_t os, int polys, void *st) {
+    return kb_bconv(c, false, x, is, y, os, polys, S(st));
+}
+extern "C" int kbfv_tensor(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out,
+                           int mod0, int limbs, int batch, void *st) {
+    return kb_tensor(c, ct1, ct2, out, mod0, limbs, batch, S(st));
+}
+extern "C" int kbfv_scale_round(const kbfv_context *c, const uint64_t *x, uint64_t *y, int polys, void *st) {
+    return kb_scale_round(c, x, y, polys, S(st));
+}
+extern "C" int kbfv_hps_decrypt(const kbfv_context *c, const uint64_t *x, uint64_t *m, int batch, void *st) {
+    return kb_hps_decrypt(c, x, m, batch, S(st));
+}
+extern "C" int kbfv_modup_qj_to_t(const kbfv_context *c, const uint64_t *x, uint64_t *out, int batch,
+                                  void *st) {
+    KB_REQUIRE(c, "null context");
+    return kb_modup_qj_to_t(c, x, (long)c->qn * c->n, out, batch, S(st));
+}
+extern "C" int kbfv_modup_pq_to_t(const kbfv_context *c, const uint64_t *x, uint64_t *out, int batch,
+                                  void *st) {
+    return kb_modup_pq_to_t(c, x, out, batch, S(st));
+}
+extern "C" int kbfv_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t *key,
+                                  uint64_t *out, int batch, void *st) {
+    return kb_inner_product(c, cin, key, out, batch, S(st));
+}
+extern "C" int kbfv_modup_t_to_pq(const kbfv_context *c, const uint64_t *in, uint64_t *out, int batch,
+                                  void *st) {
+    return kb_modup_t_to_pq(c, in, out, batch, S(st));
+}
+extern "C" int kbfv_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
+                            int64_t add_stride, int batch, void *st) {
+    return kb_moddown(c, in, out, add, add_stride, batch, S(st));
+}
+extern "C" int kbfv_xsalsa20(const uint8_t key[32], const uint8_t nonce[24], uint64_t block0, uint8_t *out,
+                             size_t len, void *st) {
+    return kb_xsalsa20(key, nonce, block0, out, len, S(st));
+}
+extern "C" int kbfv_sample(const kbfv_context *c, int kind, const uint8_t seed[32], uint64_t tag, int mod0,
+                           int limbs, uint64_t *out, void *st) {
+    return kb_sample(c, kind, seed, tag, 0, mod0, limbs, out, 0, 1, S(st));
+}
+
+// ---------------------------------------------------------------- keys
+extern "C" int kbfv_keygen_sk(const kbfv_context *c, const uint8_t seed[32], uint64_t *sk, void *st) {
+    KB_REQUIRE(c && seed && sk, "null argument");
+    const int pq = c->pn + c->qn;
+    KB_TRY(kb_sample(c, 1, seed, TAG(TAG_SK), 0, c->oP, pq, sk, 0, 1, S(st)));
+    return kb_ntt_inplace(c, +1, sk, c->oP, pq, 1, S(st));
+}
+
+extern "C" int kbfv_keygen_pk(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk, uint64_t *pk,
+                              void *st) {
+    KB_REQUIRE(c && seed && sk && pk, "null argument");
+    const long L = (long)c->qn * c->n;
+    uint64_t *a = pk, *b = pk + L;
+    const uint64_t *s_q = sk + (long)c->pn * c->n;
+    KB_TRY(kb_sample(c, 0, seed, TAG(TAG_PK_A), 0, c->oQ, c->qn, a, 0, 1, S(st)));
+    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_PK_E), 0, c->oQ, c->qn, b, 0, 1, S(st)));
+    KB_TRY(kb_ntt_inplace(c, +1, b, c->oQ, c->qn, 1, S(st)));
+    return mac_bcast(c, true, b, a, s_q, b, c->oQ, c->qn, 1, S(st));  // b = e - a*s
+}
+
+static size_t rlk_layout(const kbfv_context *c, void *base, uint64_t **s2, uint64_t **e, uint64_t **a,
+                         uint64_t **b) {
+    const size_t N = c->n, pq = c->pn + c->qn, D = c->dnum;
+    Bump bp(base);
+    uint64_t *p0 = bp.take(c->qn * N), *p1 = bp.take(D * pq * N), *p2 = bp.take(D * pq * N),
+             *p3 = bp.take(D * pq * N);
+    if (s2) *s2 = p0, *e = p1, *a = p2, *b = p3;
+    return bp.used;
+}
+
+extern "C" size_t kbfv_keygen_rlk_workspace_bytes(const kbfv_context *c) {
+    return c ? rlk_layout(c, nullptr, nullptr, nullptr, nullptr, nullptr) : 0;
+}
+
+extern "C" int kbfv_keygen_rlk(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk, uint64_t *rlk,
+                               void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && seed && sk && rlk && ws, "null argument");
+    KB_REQUIRE(ws_bytes >= kbfv_keygen_rlk_workspace_bytes(c), "workspace too small");
+    uint64_t *s2, *e, *a, *b;
+    rlk_layout(c, ws, &s2, &e, &a, &b);
+    const long N = c->n;
+    const int pn = c->pn, qn = c->qn, pq = pn + qn, D = c->dnum, B = c->ri_blocks, tn = c->tn;
+    const uint64_t *s_q = sk + (long)pn * N;
+    hipStream_t s = S(st);
+    KB_TRY(kb_poly_binop(c, 2, s_q, s_q, s2, c->oQ, qn, 1, s));
+    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_RLK_E), 0, c->oP, pq, e, 0, D, s));
+    KB_TRY(kb_ntt_inplace(c, +1, e, c->oP, pq, D, s));
+    KB_TRY(kb_sample(c, 0, seed, TAG(TAG_RLK_A), 0, c->oP, pq, a, 0, D, s));
+    const long total = (long)D * pq * N;
+    hipLaunchKernelGGL(k_rlk_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e, a, sk, s2, b, pn, qn,
+                       c->n, total);
+    KB_LAUNCH_CHECK();
+    KB_TRY(kb_ntt_inplace(c, -1, a, c->oP, pq, D, s));
+    KB_TRY(kb_ntt_inplace(c, -1, b, c->oP, pq, D, s));
+    const long half = (long)D * B * tn * N;
+    KB_TRY(kb_modup_pq_to_t(c, a, rlk, D, s));
+    KB_TRY(kb_modup_pq_to_t(c, b, rlk + half, D, s));
+    return kb_ntt_inplace(c, +1, rlk, c->oT, tn, 2 * D * B, s);
+}
+
+// ---------------------------------------------------------------- encrypt / decrypt
+static size_t enc_layout(const kbfv_context *c, int batch, void *base, uint64_t **v, uint64_t **e0,
+                         uint64_t **u1, uint64_t **dm) {
+    const size_t L = (size_t)c->qn * c->n * batch;
+    Bump bp(base);
+    uint64_t *p0 = bp.take(L), *p1 = bp.take(L), *p2 = bp.take(L), *p3 = bp.take(L);
+    if (v) *v = p0, *e0 = p1, *u1 = p2, *dm = p3;
+    return bp.used;
+}
+extern "C" size_t kbfv_encrypt_workspace_bytes(const kbfv_context *c, int batch) {
+    return (c && batch > 0) ? enc_layout(c, batch, nullptr, nullptr, nullptr, nullptr, nullptr) : 0;
+}
+
+extern "C" int kbfv_encrypt(const kbfv_context *c, const uint8_t seed[32], const uint64_t *pk, const uint64_t *m,
+                            uint64_t *ct, int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && seed && pk && m && ct && ws && batch > 0, "null argument");
+    KB_REQUIRE(ws_bytes >= kbfv_encrypt_workspace_bytes(c, batch), "workspace too small");
+    uint64_t *v, *e0, *u1, *dm;
+    enc_layout(c, batch, ws, &v, &e0, &u1, &dm);
+    const int qn = c->qn;
+    hipStream_t s = S(st);
+    KB_TRY(kb_sample(c, 2, seed, TAG(TAG_ENC_V), 0, c->oQ, qn, v, 0, batch, s));
+    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_ENC_E0), 0, c->oQ, qn, e0, 0, batch, s));
+    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_ENC_E1), 0, c->oQ, qn, u1, 0, batch, s));
+    KB_TRY(kb_delta_m(c, m, dm, batch, s));
+    KB_TRY(kb_poly_binop(c, 0, u1, dm, u1, c->oQ, qn, batch, s));  // e1 + Delta*m
+    KB_TRY(kb_ntt_inplace(c, +1, v, c->oQ, qn, batch, s));
+    KB_TRY(kb_ntt_inplace(c, +1, e0, c->oQ, qn, batch, s));
+    KB_TRY(kb_ntt_inplace(c, +1, u1, c->oQ, qn, batch, s));
+    const long total = (long)batch * qn * c->n;
+    hipLaunchKernelGGL(k_enc_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, v, e0, u1, pk, ct, qn,
+                       c->oQ, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+extern "C" size_t kbfv_decrypt_workspace_bytes(const kbfv_context *c, int batch) {
+    return (c && batch > 0) ? (size_t)c->qn * c->n * batch * sizeof(uint64_t) : 0;
+}
+
+extern "C" int kbfv_decrypt(const kbfv_context *c, const uint64_t *sk, const uint64_t *ct, uint64_t *m, int batch,
+                            void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && sk && ct && m && ws && batch > 0, "null argument");
+    KB_REQUIRE(ws_bytes >= kbfv_decrypt_workspace_bytes(c, batch), "workspace too small");
+    uint64_t *x = (uint64_t *)ws;
+    const int qn = c->qn;
+    hipStream_t s = S(st);
+    const long total = (long)batch * qn * c->n;
+    hipLaunchKernelGGL(k_dec_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, ct,
+                       sk + (long)c->pn * c->n, x, qn, c->oQ, c->n, total);
+    KB_LAUNCH_CHECK();
+    KB_TRY(kb_ntt_inplace(c, -1, x, c->oQ, qn, batch, s));
+    return kb_hps_decrypt(c, x, m, batch, s);
+}
+
+extern "C" int kbfv_add(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out, int batch,
+                        void *st) {
+    KB_REQUIRE(c && ct1 && ct2 && out && batch > 0, "null argument");
+    // [batch][2][qn][N] == [2*batch][qn][N]
+    return kb_poly_binop(c, 0, ct1, ct2, out, c->oQ, c->qn, 2 * batch, S(st));
+}
+
+// ---------------------------------------------------------------- HomMult + relinearisation
+extern "C" size_t kbfv_mult_workspace_bytes(const kbfv_context *c, int batch) {
+    return (c && batch > 0) ? mult_layout(c, batch, nullptr, nullptr) : 0;
+}
+
+extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1,
+                               const uint64_t *ct2, uint64_t *out, int batch, void *ws, size_t ws_bytes,
+                               void *st) {
+    KB_REQUIRE(c && rlk && ct1 && ct2 && out && ws && batch > 0, "null argument");
+    KB_REQUIRE(c->rn <= 64, "too many limbs");
+    // largest chunk that fits the workspace
+    int cb = batch;
+    while (cb > 1 && mult_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
+    KB_REQUIRE(mult_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
+    MultWs w;
+    mult_layout(c, cb, ws, &w);
+    const long CTS = 2L * c->qn * c->n;
+    for (int b0 = 0; b0 < batch; b0 += cb) {
+        const int n = std::min(cb, batch - b0);
+        KB_TRY(mult_chunk(c, rlk, ct1 + b0 * CTS, ct2 + b0 * CTS, out + b0 * CTS, n, w, S(st)));
+    }
+    return KBFV_OK;
+}
+
+// ---------------------------------------------------------------- encoder
+extern "C" int kbfv_encode(const kbfv_context *c, uint64_t *vals, int batch, void *st) {
+    KB_REQUIRE(c && vals && batch > 0, "null argument");
+    if (!c->batching) return KBFV_OK;  // coefficient encoding
+    return kb_ntt_inplace(c, -1, vals, c->oPlain, 1, batch, S(st));
+}
+extern "C" int kbfv_decode(const kbfv_context *c, uint64_t *vals, int batch, void *st) {
+    KB_REQUIRE(c && vals && batch > 0, "null argument");
+    KB_TRY(kb_reduce_plain(c, vals, (long)batch * c->n, S(st)));
+    if (!c->batching) return KBFV_OK;
+    return kb_ntt_inplace(c, +1, vals, c->oPlain, 1, batch, S(st));
+}
+
+// ---------------------------------------------------------------- RGSW x RLWE
+static int ndigits(const kbfv_context *c, int base_bits) {
+    int maxb = 0;
+    for (int i = 0; i < c->qn; ++i) {
+        uint64_t q = c->mod[c->oQ + i];
+        int b = 0;
+        while (q) b++, q >>= 1;
+        maxb = std::max(maxb, b);
+    }
+    return (maxb + base_bits - 1) / base_bits;
+}
+
+extern "C" int kbfv_rgsw_rows(const kbfv_context *c, int base_bits) {
+    KB_REQUIRE(c && base_bits >= 1 && base_bits <= 62, "bad gadget base");
+    return 2 * c->qn * ndigits(c, base_bits);
+}
+
+static size_t rgsw_enc_layout(const kbfv_context *c, int rows, void *base, uint64_t **e, uint64_t **mq) {
+    const size_t L = (size_t)c->qn * c->n;
+    Bump bp(base);
+    uint64_t *p0 = bp.take(L * rows), *p1 = bp.take(L);
+    if (e) *e = p0, *mq = p1;
+    return bp.used;
+}
+
+extern "C" size_t kbfv_rgsw_encrypt_workspace_bytes(const kbfv_context *c) {
+    // sized for the finest gadget (1-bit digits)
+    return c ? rgsw_enc_layout(c, 2 * c->qn * 64, nullptr, nullptr, nullptr) : 0;
+}
+
+extern "C" int kbfv_rgsw_encrypt(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk,
+                                 const uint64_t *m_coeff, int base_bits, uint64_t *rgsw, void *ws, size_t ws_bytes,
+                                 void *st) {
+    KB_REQUIRE(c && seed && sk && m_coeff && rgsw && ws, "null argument");
+    const int rows = kbfv_rgsw_rows(c, base_bits);
+    KB_REQUIRE(rows > 0, "bad gadget base");
+    KB_REQUIRE(ws_bytes >= rgsw_enc_layout(c, rows, nullptr, nullptr, nullptr), "workspace too small");
+    const int nd = ndigits(c, base_bits), qn = c->qn;
+    const long N = c->n, L = (long)qn * N;
+    uint64_t *e, *mq;
+    rgsw_enc_layout(c, rows, ws, &e, &mq);
+    hipStream_t s = S(st);
+    // row r: a_r uniform into component 0, b_r = e_r - a_r*s into component 1
+    KB_TRY(kb_sample(c, 0, seed, TAG(TAG_RGSW_A), 0, c->oQ, qn, rgsw, 2 * L, rows, s));
+    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_RGSW_E), 0, c->oQ, qn, e, L, rows, s));
+    KB_TRY(kb_ntt_inplace(c, +1, e, c->oQ, qn, rows, s));
+    const long total = (long)rows * L;
+    hipLaunchKernelGGL(k_rgsw_body, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e,
+                       sk + (long)c->pn * N, rgsw, qn, c->oQ, c->n, total);
+    KB_LAUNCH_CHECK();
+    // gadget term m * 2^(w*j) on limb i of component h
+    hipLaunchKernelGGL(k_lift_plain, dim3(blocks_for(L, BS)), dim3(BS), 0, s, c->t, m_coeff, mq, c->oQ, c->plain,
+                       c->n, L);
+    KB_LAUNCH_CHECK();
+    KB_TRY(kb_ntt_inplace(c, +1, mq, c->oQ, qn, 1, s));
+    return kb_rgsw_add_gadget(c, rgsw, mq, base_bits, nd, s);
+}
+
+extern "C" size_t kbfv_external_product_workspace_bytes(const kbfv_context *c, int batch, int base_bits) {
+    if (!c || batch <= 0) return 0;
+    const size_t L = (size_t)c->qn * c->n;
+    Bump bp(nullptr);
+    bp.take(2 * L * batch);  // coefficient form of the input
+    bp.take(8 * L * batch);  // digit chunk (up to 8 rows)
+    (void)base_bits;
+    return bp.used;
+}
+
+extern "C" int kbfv_external_product(const kbfv_context *c, const uint64_t *rgsw, const uint64_t *ct, uint64_t *out,
+                                     int batch, int base_bits, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && rgsw && ct && out && ws && batch > 0, "null argument");
+    const int rows = kbfv_rgsw_rows(c, base_bits);
+    KB_REQUIRE(rows > 0, "bad gadget base");
+    KB_REQUIRE(ws_bytes >= kbfv_external_product_workspace_bytes(c, batch, base_bits), "workspace too small");
+    const int nd = ndigits(c, base_bits), qn = c->qn;
+    const long N = c->n, L = (long)qn * N;
+    Bump bp(ws);
+    uint64_t *coef = bp.take(2 * L * batch), *dig = bp.take(8 * L * batch);
+    hipStream_t s = S(st);
+    KB_TRY(kb_ntt(c, -1, ct, coef, c->oQ, qn, 2, batch, 2 * L, L, N, 2 * L, L, N, s));
+    KB_TRY(fill_zero(out, 2 * L * batch, s));
+    for (int r0 = 0; r0 < rows; r0 += 8) {
+        const int rc = std::min(8, rows - r0);
+        KB_TRY(kb_digit_decompose(c, coef, dig, batch, base_bits, nd, r0, rc, s));
+        KB_TRY(kb_ntt(c, +1, dig, dig, c->oQ, qn, rc, batch, (long)rc * L, L, N, (long)rc * L, L, N, s));
+        KB_TRY(kb_gadget_mac(c, dig, rgsw + (long)r0 * 2 * L, out, batch, rc, s));
+    }
+    return KBFV_OK;
+}
