@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark: batched BFV HomMult + relinearisation at N=2^16 on one MI355X.
+
+Workload (one step): 128 HomMult+relin of ciphertext pairs, N = 2^16, 28 x
+60-bit ciphertext primes + 1 special prime, t = 65537 (coefficient encoding:
+65537 is not 1 mod 2^17), relinearisation key with dnum = 28 digits held in a
+4-prime auxiliary basis T (gamma = 2).  Inputs are fresh encryptions of seeded
+uniform plaintexts, resident in HBM before the timed region.
+
+Prints one JSON line: throughput of the batched path (HomMult+relin/s), the
+plain per-ciphertext path on the same inputs (128 launches of batch 1), their
+ratio, step-latency percentiles, a roofline entry for the dominant kernel
+(timed with HIP events on the library's stream) and the CPU oracle's time for
+one HomMult+relin at the same shape on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--log-n", type=int, default=16)
+    ap.add_argument("--qn", type=int, default=28)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def pct(xs, p):
+    xs = sorted(xs)
+    k = min(len(xs) - 1, max(0, int(round(p / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def cpu_baseline(ctx, ct1, ct2, rlk_shape):
+    """Oracle HomMult+relin (single thread) on one ciphertext pair of the same shape."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    import kunpbfv_amd as kb
+    oc = orc.Ctx(ctx.log_n, ctx._q, ctx._p, ctx._t, ctx.gamma, ctx.plain)
+    rng = np.random.default_rng(5)
+    # timing does not depend on the key's values: a random key of the right shape
+    rlk = np.empty(rlk_shape, dtype=np.uint64)
+    tm = oc.mod[oc.oT:oc.oT + oc.tn]
+    for k in range(oc.tn):
+        rlk[..., k, :] = rng.integers(0, int(tm[k]), size=rlk[..., k, :].shape, dtype=np.uint64)
+    a, b = kb.to_host(ct1[0]), kb.to_host(ct2[0])
+    t0 = time.perf_counter()
+    oc.mult_relin(rlk, a, b)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 4), "unit": "HomMult+relin/s", "cores": 1, "kind": "port",
+            "sample": f"1 HomMult+relin at N=2^{ctx.log_n}, {ctx.qn}+{ctx.pn} primes, CPU oracle, "
+                      f"{dt:.2f} s on one host core"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of "
+                                   "this build (the batch shards by ciphertext, see DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import kunpbfv_amd as kb
+
+    torch.cuda.init()
+    ctx = kb.Context.chain60(args.log_n, args.qn, 1, 4, 2, 65537)
+    B, N = args.batch, ctx.n
+    sk = ctx.keygen_sk(bytes([1]) * 32)
+    pk = ctx.keygen_pk(bytes([2]) * 32, sk)
+    rlk = ctx.keygen_rlk(bytes([3]) * 32, sk)
+    rng = np.random.default_rng(2024)
+    m1 = rng.integers(0, ctx.plain, size=(B, N), dtype=np.uint64)
+    m2 = rng.integers(0, ctx.plain, size=(B, N), dtype=np.uint64)
+    # entries 0 and 1 of the second batch are a monomial and a constant so that
+    # the decrypted products can be checked exactly
+    m2[0] = 0
+    m2[0, 5] = 1
+    m2[1] = 0
+    m2[1, 0] = 3
+    ct1 = ctx.encrypt(bytes([4]) * 32, pk, kb.to_device(m1))
+    ct2 = ctx.encrypt(bytes([5]) * 32, pk, kb.to_device(m2))
+    out = ctx.empty(*ct1.shape)
+    ws_bytes = ctx.mult_workspace_bytes(B)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    # ---- correctness: decrypt the two checkable products
+    ctx.mult_relin(rlk, ct1, ct2, out=out, ws=ws)
+    dec = kb.to_host(ctx.decrypt(sk, out[0:2].contiguous()))
+    want0 = np.zeros(N, dtype=np.int64)
+    want0[5:] = m1[0, :-5].astype(np.int64)
+    want0[:5] = -m1[0, -5:].astype(np.int64)
+    ok = np.array_equal(dec[0].astype(np.int64) % ctx.plain, want0 % ctx.plain)
+    ok &= np.array_equal(dec[1], (m1[1] * 3) % ctx.plain)
+    if not ok:
+        print(json.dumps({"error": "decrypted HomMult result is wrong"}))
+        sys.exit(1)
+    ref_out = out.clone()
+
+    # ---- batched path
+    for _ in range(args.warmup):
+        ctx.mult_relin(rlk, ct1, ct2, out=out, ws=ws)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        ctx.mult_relin(rlk, ct1, ct2, out=out, ws=ws)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    ms_per_step = sum(step_ms) / len(step_ms)
+    batched_ops = B / (ms_per_step / 1e3)
+    bit_exact = bool(torch.equal(out, ref_out))
+
+    # ---- plain per-ciphertext path (128 launches of batch 1) on the same inputs
+    plain_ops = None
+    plain_exact = None
+    if not args.no_plain:
+        out1 = ctx.empty(*ct1.shape)
+        ws1 = torch.empty(ctx.mult_workspace_bytes(1), dtype=torch.uint8, device="cuda")
+        c1s = [ct1[i:i + 1] for i in range(B)]
+        c2s = [ct2[i:i + 1] for i in range(B)]
+        o1s = [out1[i:i + 1] for i in range(B)]
+        for i in range(B):  # warm-up pass
+            ctx.mult_relin(rlk, c1s[i], c2s[i], out=o1s[i], ws=ws1)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.plain_steps):
+            for i in range(B):
+                ctx.mult_relin(rlk, c1s[i], c2s[i], out=o1s[i], ws=ws1)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1) / args.plain_steps
+        plain_ops = B / (plain_ms / 1e3)
+        plain_exact = bool(torch.equal(out1, ref_out))
+
+    # ---- dominant kernel: the forward NTT over the QR basis as the pipeline runs it
+    # (inverse QR transform of the 3 tensor polynomials of every ciphertext)
+    roof = kernel_roofline(ctx, B, stream)
+
+    result = {
+        "metric": "HomMult+relin/s at N=2^16 batched vs plain (bit-exact); p50/p99 latency",
+        "value": round(batched_ops, 2),
+        "unit": "HomMult+relin/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded uniform plaintexts mod t, fresh encryptions)",
+        "config": {"workload": f"batched HomMult+relin, N=2^{args.log_n}, {args.qn}+1 x 60-bit primes, "
+                               f"t=65537, dnum={ctx.dnum}, T basis {ctx.tn} primes (gamma={ctx.gamma}), "
+                               f"batch {B}",
+                   "batch": B, "log_n": args.log_n, "q_primes": args.qn, "special_primes": 1},
+        "plain_path_ops_per_s": round(plain_ops, 2) if plain_ops else None,
+        "batched_over_plain": round(batched_ops / plain_ops, 3) if plain_ops else None,
+        "bit_exact_batched_vs_plain": plain_exact,
+        "bit_exact_across_steps": bit_exact,
+        "step_latency_ms": {"p50": round(pct(step_ms, 50), 3), "p99": round(pct(step_ms, 99), 3)},
+        "wall_s": round(wall, 3),
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(ctx, ct1, ct2, tuple(rlk.shape))
+    print(json.dumps(result), flush=True)
+
+
+def kernel_roofline(ctx, B, stream):
+    """Average duration of one launch of the column-pass INTT kernel over the QR
+    basis for the whole batch, measured with HIP events on the stream the
+    library launches on, against its algorithmic HBM bytes."""
+    import torch
+    import kunpbfv_amd as kb
+    sn = ctx.qn + ctx.rn
+    polys = 3 * B
+    x = ctx.empty(polys, sn, ctx.n)
+    x.random_(0, 1 << 40)
+    reps = 5
+    # one inverse NTT = row pass + column pass; time the pair and report per launch
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ctx.ntt(x, x, -1, ctx.oQ, sn, polys=polys)
+    torch.cuda.synchronize()
+    evs[0].record(stream)
+    for _ in range(reps):
+        ctx.ntt(x, x, -1, ctx.oQ, sn, polys=polys)
+    evs[1].record(stream)
+    torch.cuda.synchronize()
+    per_transform_s = evs[0].elapsed_time(evs[1]) / reps / 1e3
+    per_launch_s = per_transform_s / 2  # two passes
+    bytes_per_launch = polys * sn * ctx.n * 8 * 2  # read + write every limb once per pass
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    return {"kernel": "k_ntt_pass (inverse, QR basis, whole batch)", "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "launch_ms": round(per_launch_s * 1e3, 3), "bytes_per_launch": bytes_per_launch}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+// C++ drop-in surface of the kunpbfv engine: the classes a caller of the
+// reference library (QiiPatri/kunpBFV) uses, with the same names, members and
+// argument meaning, implemented on the C ABI of libkunpbfv.so (kunpbfv.h).
+//
+// Interfaces kept (reference file:line):
+//   BFVContext    include/BFVcontext.h:17-391   (ctor, encode/encode_ntt/decode,
+//                                                ToNTTInplace/FromNTTInplace, public counts)
+//   BFVScheme     include/BFVScheme.h:12-179    (mallocMemory, addEncKey, addMultKey_23,
+//                                                encryptMsg, decryptMsg, add*, mult*_23,
+//                                                multConstAndEqual)
+//   Ciphertext    include/Ciphertext.cuh:5-89   (cipher_device = [ax | bx], N, L, l, slots)
+//   Plaintext     include/Plaintext.cuh:6-69    (mx_device, N, L, l, slots)
+//   SecretKey     include/SecretKey.cuh:10-63   (sx_device over K+L+1 limbs, NTT form)
+//   Key           include/Key.cuh:6-81          (public key [ax | bx])
+//   Key_decomp    include/Key_decomp.cuh:6-70   (relinearisation key, [2][dnum][blocks][T][N])
+// Errors: std::invalid_argument for level-0 ciphertexts and a missing
+// relinearisation key (include/KeySwitch_23.cuh:246-253), std::runtime_error for
+// a failed device call (the reference never checks them).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <vector>
+
+#include "../kunpbfv.h"
+
+typedef unsigned long long uint64_tt;
+typedef unsigned int uint32_tt;
+typedef unsigned char uint8_tt;
+
+class Plaintext {
+   public:
+    uint64_tt *mx_device = nullptr;
+    int N = 0, L = 0, l = 0, slots = 0;
+    Plaintext() = default;
+    Plaintext(int N, int L, int l, int slots);
+    Plaintext(const Plaintext &o);
+    Plaintext &operator=(const Plaintext &o);
+    ~Plaintext();
+};
+
+class Ciphertext {
+   public:
+    uint64_tt *cipher_device = nullptr;
+    uint64_tt *ax_device = nullptr;
+    uint64_tt *bx_device = nullptr;
+    int N = 0, l = 0, L = 0, slots = 0;
+    Ciphertext() = default;
+    Ciphertext(int N, int L, int l, int slots);
+    Ciphertext(const Ciphertext &o);
+    Ciphertext &operator=(const Ciphertext &o);
+    ~Ciphertext();
+};
+
+class BFVContext {
+   public:
+    // default parameter set of the ring degree (logN 12..16)
+    explicit BFVContext(size_t poly_modulus_degree);
+    // explicit chain: q (ciphertext), p (special), t (auxiliary key-switch basis)
+    BFVContext(int logN, const std::vector<uint64_tt> &q, const std::vector<uint64_tt> &p,
+               const std::vector<uint64_tt> &t, int gamma, uint64_tt plain_modulus);
+    ~BFVContext();
+    BFVContext(const BFVContext &) = delete;
+    BFVContext &operator=(const BFVContext &) = delete;
+
+    // slots -> plaintext: INTT over t in place on `vals` (device, N values), then
+    // the coefficients replicated into every limb of msg (include/Encoder.cuh:23)
+    void encode(uint64_tt *vals, Plaintext &msg);
+    // coefficients replicated into every limb and NTT'd over Q (include/Encoder.cuh:64)
+    void encode_ntt(uint64_tt *vals, Plaintext &msg);
+    // limb 0 of msg -> vals, NTT over t (include/Encoder.cuh:81)
+    void decode(Plaintext &msg, uint64_tt *vals);
+    // forward / inverse NTT of poly_num polynomials of poly_mod_len limbs, limbs
+    // [start_mod_idx, start_mod_idx + mod_num) of the [P | Q] basis
+    // (include/BFVcontext.cuh:1513-1552)
+    void ToNTTInplace(uint64_tt *device_a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num,
+                      int poly_mod_len);
+    void FromNTTInplace(uint64_tt *device_a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num,
+                        int poly_mod_len);
+
+    int logN, logNh, logslots, slots, L, q_num, K, dnum, alpha, p_num, t_num, r_num, mod_num, gamma;
+    int Ri_blockNum, Qj_blockNum;
+    long N, M, Nh;
+    uint64_tt plain_modulus;
+    std::vector<uint64_tt> qVec, pVec, tVec, rVec;
+
+    kbfv_context *handle = nullptr;
+    hipStream_t stream = nullptr;
+
+   private:
+    void init(int logN, const std::vector<uint64_tt> &q, const std::vector<uint64_tt> &p,
+              const std::vector<uint64_tt> &t, int gamma, uint64_tt plain);
+};
+
+class SecretKey {
+   public:
+    uint64_tt *sx_device = nullptr;
+    int N = 0, L = 0, K = 0;
+    explicit SecretKey(BFVContext &context, hipStream_t stream = 0);
+    SecretKey(BFVContext &context, const uint8_t seed[32], hipStream_t stream = 0);
+    ~SecretKey();
+    SecretKey(const SecretKey &) = delete;
+    SecretKey &operator=(const SecretKey &) = delete;
+};
+
+class Key {
+   public:
+    uint64_tt *cipher_device = nullptr, *ax_device = nullptr, *bx_device = nullptr;
+    int N = 0, L = 0;
+    Key(int N, int L);
+    ~Key();
+    Key(const Key &) = delete;
+    Key &operator=(const Key &) = delete;
+};
+
+class Key_decomp {
+   public:
+    uint64_tt *cipher_device = nullptr, *ax_device = nullptr, *bx_device = nullptr;
+    int N = 0, dnum = 0, t_num = 0, Ri_blockNum = 0;
+    Key_decomp(int N, int dnum, int t_num, int Ri_blockNum);
+    ~Key_decomp();
+    Key_decomp(const Key_decomp &) = delete;
+    Key_decomp &operator=(const Key_decomp &) = delete;
+};
+
+class BFVScheme {
+   public:
+    explicit BFVScheme(BFVContext &context);
+    ~BFVScheme();
+    BFVScheme(const BFVScheme &) = delete;
+    BFVScheme &operator=(const BFVScheme &) = delete;
+
+    // scratch for single-ciphertext operations (grows on demand for batches)
+    void mallocMemory();
+    // deterministic randomness: every key / encryption draws its seed from here
+    void setSeed(uint64_t seed);
+
+    void addEncKey(SecretKey &secretKey, hipStream_t stream = 0);
+    void addMultKey_23(SecretKey &secretKey, hipStream_t stream = 0);
+    // rotation keys are outside this engine's scope: accepted and ignored
+    void addLeftRotKey_23(SecretKey &secretKey, long rot, hipStream_t stream = 0);
+
+    void encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t stream = 0);
+    void decryptMsg(Plaintext &plain, SecretKey &secretKey, Ciphertext &cipher, hipStream_t stream = 0);
+
+    void add(Ciphertext &cipher_res, Ciphertext &cipher1, Ciphertext &cipher2);
+    void addAndEqual(Ciphertext &cipher1, Ciphertext &cipher2);
+    void sub(Ciphertext &cipher_res, Ciphertext &cipher1, Ciphertext &cipher2);
+    void subAndEqual(Ciphertext &cipher1, Ciphertext &cipher2);
+
+    void mult_23(Ciphertext &cipher_res, Ciphertext &cipher1, Ciphertext &cipher2);
+    void multAndEqual_23(Ciphertext &cipher1, Ciphertext &cipher2);
+    // the batched form: every kernel covers all pairs, the key is read once
+    void multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector<Ciphertext *> &c2);
+    // plaintext (from encode_ntt) times ciphertext (include/BFVScheme.cuh:226)
+    void multConstAndEqual(Ciphertext &cipher, Plaintext &cnst);
+
+    BFVContext &context;
+    Key *publicKey = nullptr;
+    Key_decomp *rlk_23 = nullptr;
+
+   private:
+    void seed_next(uint8_t out[32]);
+    void *ensure_ws(size_t bytes);
+    uint64_t seed_state = 0x6b756e7042465600ULL;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+};

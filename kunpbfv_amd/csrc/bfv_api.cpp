@@ -1,0 +1,370 @@
+// C++ drop-in classes (include/kunpbfv/bfv.hpp) on the C ABI.
+#include "kunpbfv/bfv.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+namespace {
+
+void ok(int rc, const char *what) {
+    if (rc != KBFV_OK) throw std::runtime_error(std::string(what) + ": " + kbfv_last_error());
+}
+void hip_ok(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+uint64_tt *dalloc(size_t elems) {
+    void *p = nullptr;
+    hip_ok(hipMalloc(&p, elems * sizeof(uint64_tt)), "hipMalloc");
+    return (uint64_tt *)p;
+}
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- Plaintext / Ciphertext
+Plaintext::Plaintext(int N_, int L_, int l_, int slots_) : N(N_), L(L_), l(l_), slots(slots_) {
+    mx_device = dalloc((size_t)N * (L + 1));
+    hip_ok(hipMemset(mx_device, 0, sizeof(uint64_tt) * N * (L + 1)), "hipMemset");
+}
+Plaintext::Plaintext(const Plaintext &o) : N(o.N), L(o.L), l(o.l), slots(o.slots) {
+    mx_device = dalloc((size_t)N * (L + 1));
+    hip_ok(hipMemcpy(mx_device, o.mx_device, sizeof(uint64_tt) * N * (L + 1), hipMemcpyDeviceToDevice), "copy");
+}
+Plaintext &Plaintext::operator=(const Plaintext &o) {
+    if (this == &o) return *this;
+    if (N != o.N || L != o.L) {
+        dfree(mx_device);
+        N = o.N, L = o.L;
+        mx_device = dalloc((size_t)N * (L + 1));
+    }
+    l = o.l, slots = o.slots;
+    hip_ok(hipMemcpy(mx_device, o.mx_device, sizeof(uint64_tt) * N * (L + 1), hipMemcpyDeviceToDevice), "copy");
+    return *this;
+}
+Plaintext::~Plaintext() { dfree(mx_device); }
+
+Ciphertext::Ciphertext(int N_, int L_, int l_, int slots_) : N(N_), l(l_), L(L_), slots(slots_) {
+    cipher_device = dalloc((size_t)2 * N * (L + 1));
+    ax_device = cipher_device;
+    bx_device = cipher_device + (size_t)N * (L + 1);
+}
+Ciphertext::Ciphertext(const Ciphertext &o) : N(o.N), l(o.l), L(o.L), slots(o.slots) {
+    cipher_device = dalloc((size_t)2 * N * (L + 1));
+    ax_device = cipher_device;
+    bx_device = cipher_device + (size_t)N * (L + 1);
+    hip_ok(hipMemcpy(cipher_device, o.cipher_device, sizeof(uint64_tt) * 2 * N * (L + 1), hipMemcpyDeviceToDevice),
+           "copy");
+}
+Ciphertext &Ciphertext::operator=(const Ciphertext &o) {
+    if (this == &o) return *this;
+    if (N != o.N || L != o.L) {
+        dfree(cipher_device);
+        N = o.N, L = o.L;
+        cipher_device = dalloc((size_t)2 * N * (L + 1));
+        ax_device = cipher_device;
+        bx_device = cipher_device + (size_t)N * (L + 1);
+    }
+    l = o.l, slots = o.slots;
+    hip_ok(hipMemcpy(cipher_device, o.cipher_device, sizeof(uint64_tt) * 2 * N * (L + 1), hipMemcpyDeviceToDevice),
+           "copy");
+    return *this;
+}
+Ciphertext::~Ciphertext() { dfree(cipher_device); }
+
+// ---------------------------------------------------------------- BFVContext
+BFVContext::BFVContext(size_t poly_modulus_degree) {
+    int log_n = 0;
+    while ((size_t(1) << log_n) < poly_modulus_degree) log_n++;
+    if ((size_t(1) << log_n) != poly_modulus_degree) throw std::invalid_argument("N must be a power of two");
+    uint64_t q[64], p[64], t[64], plain;
+    int qn, pn, tn, g;
+    ok(kbfv_default_params(log_n, q, &qn, p, &pn, t, &tn, &g, &plain), "default parameters");
+    init(log_n, std::vector<uint64_tt>(q, q + qn), std::vector<uint64_tt>(p, p + pn),
+         std::vector<uint64_tt>(t, t + tn), g, plain);
+}
+
+BFVContext::BFVContext(int log_n, const std::vector<uint64_tt> &q, const std::vector<uint64_tt> &p,
+                       const std::vector<uint64_tt> &t, int g, uint64_tt plain) {
+    init(log_n, q, p, t, g, plain);
+}
+
+void BFVContext::init(int log_n, const std::vector<uint64_tt> &q, const std::vector<uint64_tt> &p,
+                      const std::vector<uint64_tt> &t, int g, uint64_tt plain) {
+    ok(kbfv_context_create(log_n, (const uint64_t *)q.data(), (int)q.size(), (const uint64_t *)p.data(),
+                           (int)p.size(), (const uint64_t *)t.data(), (int)t.size(), g, plain, &handle),
+       "context");
+    int info[16];
+    ok(kbfv_context_info(handle, info), "context info");
+    logN = info[0];
+    N = info[1];
+    q_num = info[2], p_num = info[3], t_num = info[4], r_num = info[5], gamma = info[6], dnum = info[7];
+    Ri_blockNum = info[8];
+    K = p_num, alpha = p_num, L = q_num - 1;
+    Qj_blockNum = (q_num + p_num - 1) / p_num;
+    mod_num = p_num + q_num + t_num;
+    logNh = logN - 1, logslots = logN, slots = (int)N, M = 2 * N, Nh = N / 2;
+    plain_modulus = plain;
+    qVec = q, pVec = p, tVec = t;
+    std::vector<uint64_t> mods(info[10]);
+    kbfv_context_moduli(handle, mods.data());
+    rVec.assign(mods.begin() + info[13], mods.begin() + info[13] + r_num);
+}
+
+BFVContext::~BFVContext() { kbfv_context_destroy(handle); }
+
+void BFVContext::encode(uint64_tt *vals, Plaintext &msg) {
+    ok(kbfv_encode(handle, (uint64_t *)vals, 1, stream), "encode");
+    for (int i = 0; i <= L; ++i)
+        hip_ok(hipMemcpyAsync(msg.mx_device + (size_t)i * N, vals, sizeof(uint64_tt) * N, hipMemcpyDeviceToDevice,
+                              stream),
+               "encode copy");
+    hip_ok(hipStreamSynchronize(stream), "encode");
+}
+
+void BFVContext::encode_ntt(uint64_tt *vals, Plaintext &msg) {
+    for (int i = 0; i <= L; ++i)
+        hip_ok(hipMemcpyAsync(msg.mx_device + (size_t)i * N, vals, sizeof(uint64_tt) * N, hipMemcpyDeviceToDevice,
+                              stream),
+               "encode_ntt copy");
+    const int64_t n = N;
+    // modulus index of q_0 is p_num (the table is [P | Q | R | T | t])
+    ok(kbfv_ntt(handle, +1, (uint64_t *)msg.mx_device, (uint64_t *)msg.mx_device, p_num, msg.l + 1, 1, 1, 0,
+                (L + 1) * n, n, 0, (L + 1) * n, n, stream),
+       "encode_ntt");
+    hip_ok(hipStreamSynchronize(stream), "encode_ntt");
+}
+
+void BFVContext::decode(Plaintext &msg, uint64_tt *vals) {
+    hip_ok(hipMemcpyAsync(vals, msg.mx_device, sizeof(uint64_tt) * N, hipMemcpyDeviceToDevice, stream), "decode");
+    ok(kbfv_decode(handle, (uint64_t *)vals, 1, stream), "decode");
+    hip_ok(hipStreamSynchronize(stream), "decode");
+}
+
+// the reference indexes moduli as [P | Q | T]; this engine's table is [P | Q | R | T | t]
+static int pqt_to_table(const BFVContext &c, int idx) {
+    return idx >= c.p_num + c.q_num ? idx + c.r_num : idx;
+}
+
+void BFVContext::ToNTTInplace(uint64_tt *a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num_,
+                              int poly_mod_len) {
+    const int64_t n = N;
+    start_mod_idx = pqt_to_table(*this, start_mod_idx);
+    uint64_t *base = (uint64_t *)a + (int64_t)start_poly_idx * n;
+    ok(kbfv_ntt(handle, +1, base, base, start_mod_idx, mod_num_, poly_num, 1, 0, poly_mod_len * n, n, 0,
+                poly_mod_len * n, n, stream),
+       "ToNTTInplace");
+}
+
+void BFVContext::FromNTTInplace(uint64_tt *a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num_,
+                                int poly_mod_len) {
+    const int64_t n = N;
+    start_mod_idx = pqt_to_table(*this, start_mod_idx);
+    uint64_t *base = (uint64_t *)a + (int64_t)start_poly_idx * n;
+    ok(kbfv_ntt(handle, -1, base, base, start_mod_idx, mod_num_, poly_num, 1, 0, poly_mod_len * n, n, 0,
+                poly_mod_len * n, n, stream),
+       "FromNTTInplace");
+}
+
+// ---------------------------------------------------------------- keys
+static uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+SecretKey::SecretKey(BFVContext &context, hipStream_t stream) : N((int)context.N), L(context.L), K(context.K) {
+    uint64_t s = 0x5345435245544b59ULL;  // fixed default: keys are reproducible run to run
+    uint8_t seed[32];
+    for (int i = 0; i < 4; ++i) {
+        uint64_t v = splitmix(s);
+        memcpy(seed + 8 * i, &v, 8);
+    }
+    sx_device = dalloc((size_t)N * (K + L + 1));
+    ok(kbfv_keygen_sk(context.handle, seed, (uint64_t *)sx_device, stream), "secret key");
+    hip_ok(hipStreamSynchronize(stream), "secret key");
+}
+
+SecretKey::SecretKey(BFVContext &context, const uint8_t seed[32], hipStream_t stream)
+    : N((int)context.N), L(context.L), K(context.K) {
+    sx_device = dalloc((size_t)N * (K + L + 1));
+    ok(kbfv_keygen_sk(context.handle, seed, (uint64_t *)sx_device, stream), "secret key");
+    hip_ok(hipStreamSynchronize(stream), "secret key");
+}
+SecretKey::~SecretKey() { dfree(sx_device); }
+
+Key::Key(int N_, int L_) : N(N_), L(L_) {
+    cipher_device = dalloc((size_t)2 * N * (L + 1));
+    ax_device = cipher_device;
+    bx_device = cipher_device + (size_t)N * (L + 1);
+}
+Key::~Key() { dfree(cipher_device); }
+
+Key_decomp::Key_decomp(int N_, int dnum_, int t_num_, int blocks) : N(N_), dnum(dnum_), t_num(t_num_), Ri_blockNum(blocks) {
+    const size_t half = (size_t)dnum * blocks * t_num * N;
+    cipher_device = dalloc(2 * half);
+    ax_device = cipher_device;
+    bx_device = cipher_device + half;
+}
+Key_decomp::~Key_decomp() { dfree(cipher_device); }
+
+// ---------------------------------------------------------------- BFVScheme
+BFVScheme::BFVScheme(BFVContext &ctx) : context(ctx) {}
+BFVScheme::~BFVScheme() {
+    delete publicKey;
+    delete rlk_23;
+    dfree(ws);
+}
+
+void BFVScheme::setSeed(uint64_t seed) { seed_state = seed; }
+
+void BFVScheme::seed_next(uint8_t out[32]) {
+    for (int i = 0; i < 4; ++i) {
+        uint64_t v = splitmix(seed_state);
+        memcpy(out + 8 * i, &v, 8);
+    }
+}
+
+void *BFVScheme::ensure_ws(size_t bytes) {
+    if (bytes > ws_bytes) {
+        dfree(ws);
+        ws = nullptr;
+        hip_ok(hipMalloc(&ws, bytes), "workspace");
+        ws_bytes = bytes;
+    }
+    return ws;
+}
+
+void BFVScheme::mallocMemory() {
+    kbfv_context *h = context.handle;
+    size_t need = kbfv_mult_workspace_bytes(h, 1);
+    need = std::max(need, kbfv_encrypt_workspace_bytes(h, 1));
+    need = std::max(need, kbfv_decrypt_workspace_bytes(h, 1));
+    need = std::max(need, kbfv_keygen_rlk_workspace_bytes(h));
+    ensure_ws(need);
+}
+
+void BFVScheme::addEncKey(SecretKey &sk, hipStream_t stream) {
+    if (!publicKey) publicKey = new Key((int)context.N, context.L);
+    uint8_t seed[32];
+    seed_next(seed);
+    ok(kbfv_keygen_pk(context.handle, seed, (uint64_t *)sk.sx_device, (uint64_t *)publicKey->cipher_device, stream),
+       "public key");
+    hip_ok(hipStreamSynchronize(stream), "public key");
+}
+
+void BFVScheme::addMultKey_23(SecretKey &sk, hipStream_t stream) {
+    if (rlk_23) return;  // as the reference: generated once
+    rlk_23 = new Key_decomp((int)context.N, context.dnum, context.t_num, context.Ri_blockNum);
+    uint8_t seed[32];
+    seed_next(seed);
+    void *w = ensure_ws(kbfv_keygen_rlk_workspace_bytes(context.handle));
+    ok(kbfv_keygen_rlk(context.handle, seed, (uint64_t *)sk.sx_device, (uint64_t *)rlk_23->cipher_device, w,
+                       ws_bytes, stream),
+       "relinearisation key");
+    hip_ok(hipStreamSynchronize(stream), "relinearisation key");
+}
+
+void BFVScheme::addLeftRotKey_23(SecretKey &, long, hipStream_t) {
+    static bool warned = false;
+    if (!warned) {
+        fprintf(stderr, "kunpbfv: rotation keys are not supported; addLeftRotKey_23 ignored\n");
+        warned = true;
+    }
+}
+
+void BFVScheme::encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t stream) {
+    if (!publicKey) throw std::invalid_argument("public key not generated (call addEncKey)");
+    cipher.l = message.l;
+    uint8_t seed[32];
+    seed_next(seed);
+    void *w = ensure_ws(kbfv_encrypt_workspace_bytes(context.handle, 1));
+    ok(kbfv_encrypt(context.handle, seed, (uint64_t *)publicKey->cipher_device, (uint64_t *)message.mx_device,
+                    (uint64_t *)cipher.cipher_device, 1, w, ws_bytes, stream),
+       "encrypt");
+    hip_ok(hipStreamSynchronize(stream), "encrypt");
+}
+
+void BFVScheme::decryptMsg(Plaintext &plain, SecretKey &sk, Ciphertext &cipher, hipStream_t stream) {
+    plain.l = cipher.l;
+    plain.slots = cipher.slots;
+    void *w = ensure_ws(kbfv_decrypt_workspace_bytes(context.handle, 1));
+    ok(kbfv_decrypt(context.handle, (uint64_t *)sk.sx_device, (uint64_t *)cipher.cipher_device,
+                    (uint64_t *)plain.mx_device, 1, w, ws_bytes, stream),
+       "decrypt");
+    // the result is replicated into every limb, as the reference's HPS decrypt does
+    for (int i = 1; i <= plain.L; ++i)
+        hip_ok(hipMemcpyAsync(plain.mx_device + (size_t)i * plain.N, plain.mx_device, sizeof(uint64_tt) * plain.N,
+                              hipMemcpyDeviceToDevice, stream),
+               "decrypt copy");
+    hip_ok(hipStreamSynchronize(stream), "decrypt");
+}
+
+void BFVScheme::add(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
+    res.l = std::min(c1.l, c2.l);
+    ok(kbfv_add(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
+                (uint64_t *)res.cipher_device, 1, context.stream),
+       "add");
+}
+void BFVScheme::addAndEqual(Ciphertext &c1, Ciphertext &c2) { add(c1, c1, c2); }
+void BFVScheme::sub(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
+    res.l = std::min(c1.l, c2.l);
+    ok(kbfv_poly_sub(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
+                     (uint64_t *)res.cipher_device, context.p_num, context.q_num, 2, context.stream),
+       "sub");
+}
+void BFVScheme::subAndEqual(Ciphertext &c1, Ciphertext &c2) { sub(c1, c1, c2); }
+
+void BFVScheme::mult_23(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
+    if (c1.l == 0 || c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    if (!rlk_23) throw std::invalid_argument("rotKey_23 not exists");
+    res.l = std::min(c1.l, c2.l);
+    void *w = ensure_ws(kbfv_mult_workspace_bytes(context.handle, 1));
+    ok(kbfv_mult_relin(context.handle, (uint64_t *)rlk_23->cipher_device, (uint64_t *)c1.cipher_device,
+                       (uint64_t *)c2.cipher_device, (uint64_t *)res.cipher_device, 1, w, ws_bytes, context.stream),
+       "mult");
+    hip_ok(hipStreamSynchronize(context.stream), "mult");
+}
+
+void BFVScheme::multAndEqual_23(Ciphertext &c1, Ciphertext &c2) { mult_23(c1, c1, c2); }
+
+void BFVScheme::multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector<Ciphertext *> &c2) {
+    if (c1.size() != c2.size()) throw std::invalid_argument("batch sizes differ");
+    if (!rlk_23) throw std::invalid_argument("rotKey_23 not exists");
+    const int B = (int)c1.size();
+    if (B == 0) return;
+    const size_t ct = (size_t)2 * context.N * (context.L + 1);
+    // gather into contiguous batches, run one batched call, scatter back
+    uint64_tt *a = dalloc(ct * B), *b = dalloc(ct * B);
+    for (int i = 0; i < B; ++i) {
+        if (c1[i]->l == 0 || c2[i]->l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+        hip_ok(hipMemcpyAsync(a + i * ct, c1[i]->cipher_device, ct * 8, hipMemcpyDeviceToDevice, context.stream), "gather");
+        hip_ok(hipMemcpyAsync(b + i * ct, c2[i]->cipher_device, ct * 8, hipMemcpyDeviceToDevice, context.stream), "gather");
+    }
+    void *w = ensure_ws(kbfv_mult_workspace_bytes(context.handle, B));
+    ok(kbfv_mult_relin(context.handle, (uint64_t *)rlk_23->cipher_device, (uint64_t *)a, (uint64_t *)b,
+                       (uint64_t *)a, B, w, ws_bytes, context.stream),
+       "batched mult");
+    for (int i = 0; i < B; ++i)
+        hip_ok(hipMemcpyAsync(c1[i]->cipher_device, a + i * ct, ct * 8, hipMemcpyDeviceToDevice, context.stream), "scatter");
+    hip_ok(hipStreamSynchronize(context.stream), "batched mult");
+    dfree(a);
+    dfree(b);
+}
+
+void BFVScheme::multConstAndEqual(Ciphertext &cipher, Plaintext &cnst) {
+    if (cipher.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    const size_t n = (size_t)context.N * (context.L + 1);
+    ok(kbfv_poly_mul(context.handle, (uint64_t *)cipher.ax_device, (uint64_t *)cnst.mx_device,
+                     (uint64_t *)cipher.ax_device, context.p_num, context.q_num, 1, context.stream),
+       "multConst");
+    ok(kbfv_poly_mul(context.handle, (uint64_t *)cipher.bx_device, (uint64_t *)cnst.mx_device,
+                     (uint64_t *)cipher.bx_device, context.p_num, context.q_num, 1, context.stream),
+       "multConst");
+    (void)n;
+    hip_ok(hipStreamSynchronize(context.stream), "multConst");
+}

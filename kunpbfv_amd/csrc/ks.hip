@@ -90,47 +90,78 @@ __global__ void k_modup_pq_to_t(kb_dev_tables t, const uint64_t *__restrict__ x,
 }
 
 // Key-stationary inner product.  cin [batch][D][tn][N], key [2][D][B][tn][N],
-// out [batch][2][B][tn][N].  Thread = (coefficient, T limb k, block b).
+// out [batch][2][B][tn][N].  A workgroup owns one T limb, 64 coefficients and
+// up to IP_WAVES key blocks (one wave per block b, both key halves in
+// registers).  It walks the whole batch: per ciphertext the D x 64 digit tile
+// is staged once in LDS and read by every wave, so the key is read from HBM
+// once per call and each digit tile ceil(B / IP_WAVES) times.
+constexpr int IP_WAVES = 8;
 template <int MAXD>
-__global__ void __launch_bounds__(KS_BS)
+__global__ void __launch_bounds__(64 * IP_WAVES)
     k_inner_product(kb_dev_tables t, const uint64_t *__restrict__ cin, const uint64_t *__restrict__ key,
                     uint64_t *__restrict__ out, int D, int B, int tn, int oT, int n, int batch) {
-    // block index -> (b fastest, then coefficient tile, then T limb): the B
-    // workgroups that share one slice of cin run side by side.
-    const int tiles = n / KS_BS;
+    __shared__ uint64_t cs[MAXD][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int bgroups = (B + IP_WAVES - 1) / IP_WAVES;
+    const int tiles = n / 64;
     int bid = blockIdx.x;
-    const int b = bid % B;
-    bid /= B;
+    const int bg = bid % bgroups;
+    bid /= bgroups;
     const int tile = bid % tiles;
     const int kk = bid / tiles;
-    const int k = tile * KS_BS + threadIdx.x;
+    const int b = bg * IP_WAVES + wave;
+    const bool active = b < B;
+    const int k = tile * 64 + lane;
     const int m = oT + kk;
     const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
     const long N = n;
-    uint64_t ka[MAXD], kbv[MAXD];
     const long kstride = (long)B * tn * N;  // digit stride in the key
+    uint64_t ka[MAXD], kbv[MAXD];
 #pragma unroll
     for (int i = 0; i < MAXD; ++i) {
-        if (i < D) {
+        ka[i] = 0;
+        kbv[i] = 0;
+        if (active && i < D) {
             const long off = (long)i * kstride + ((long)b * tn + kk) * N + k;
             ka[i] = key[off];
             kbv[i] = key[(long)D * kstride + off];
         }
     }
-    for (int bt = 0; bt < batch; ++bt) {
-        const uint64_t *cp = cin + ((long)bt * D * tn + kk) * N + k;
-        kbfv_u128 a0{0, 0}, a1{0, 0};
+    // each thread stages at most ceil(MAXD*64 / (64*IP_WAVES)) digit values per ciphertext
+    constexpr int PER = (MAXD + IP_WAVES - 1) / IP_WAVES;
+    uint64_t pre[PER];
+    auto fetch = [&](int bt) {
 #pragma unroll
-        for (int i = 0; i < MAXD; ++i) {
-            if (i < D) {
-                const uint64_t cv = cp[(long)i * tn * N];
-                kb_mac128(a0, cv, ka[i]);
-                kb_mac128(a1, cv, kbv[i]);
-            }
+        for (int r = 0; r < PER; ++r) {
+            const int i = wave + r * IP_WAVES;
+            if (i < D) pre[r] = cin[(((long)bt * D + i) * tn + kk) * N + tile * 64 + lane];
         }
-        uint64_t *op = out + (long)bt * 2 * B * tn * N + ((long)b * tn + kk) * N + k;
-        op[0] = kb_barrett128(a0, q, mlo, mhi);
-        op[(long)B * tn * N] = kb_barrett128(a1, q, mlo, mhi);
+    };
+    fetch(0);
+    for (int bt = 0; bt < batch; ++bt) {
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int i = wave + r * IP_WAVES;
+            if (i < D) cs[i][lane] = pre[r];
+        }
+        __syncthreads();
+        if (bt + 1 < batch) fetch(bt + 1);
+        if (active) {
+            kbfv_u128 a0{0, 0}, a1{0, 0};
+#pragma unroll
+            for (int i = 0; i < MAXD; ++i) {
+                if (i < D) {
+                    const uint64_t cv = cs[i][lane];
+                    kb_mac128(a0, cv, ka[i]);
+                    kb_mac128(a1, cv, kbv[i]);
+                }
+            }
+            uint64_t *op = out + (long)bt * 2 * B * tn * N + ((long)b * tn + kk) * N + k;
+            op[0] = kb_barrett128(a0, q, mlo, mhi);
+            op[(long)B * tn * N] = kb_barrett128(a1, q, mlo, mhi);
+        }
+        __syncthreads();
     }
 }
 
@@ -254,10 +285,10 @@ int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t 
                      int batch, hipStream_t st) {
     KB_REQUIRE(c && cin && key && out && batch > 0, "bad inner-product arguments");
     const int D = c->dnum, B = c->ri_blocks, tn = c->tn, n = c->n;
-    const unsigned grid = (unsigned)((n / KS_BS) * tn * B);
-#define KB_IP(MD)                                                                                    \
-    hipLaunchKernelGGL(k_inner_product<MD>, dim3(grid), dim3(KS_BS), 0, st, c->t, cin, key, out, D, B, \
-                       tn, c->oT, n, batch)
+    const unsigned grid = (unsigned)((n / 64) * tn * ((B + IP_WAVES - 1) / IP_WAVES));
+#define KB_IP(MD)                                                                                        \
+    hipLaunchKernelGGL(k_inner_product<MD>, dim3(grid), dim3(64 * IP_WAVES), 0, st, c->t, cin, key, out, D, \
+                       B, tn, c->oT, n, batch)
     if (D <= 4)
         KB_IP(4);
     else if (D <= 8)

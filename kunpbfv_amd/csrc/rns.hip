@@ -45,6 +45,10 @@ __global__ void k_poly_binop(kb_dev_tables t, const uint64_t *__restrict__ a,
 
 // ------------------------------------------------------------------ HPS base conversion
 // x: [polys][in_n][N] (stride in_stride), y: [polys][out_n][N] (stride out_stride)
+// Phase 1 (all waves): y_i = [x_i * (Q/q_i)^-1]_{q_i} into LDS.  Phase 2 (wave 0):
+// the reference's double estimate of the overflow count, summed in input order.
+// Phase 3 (all waves, one output limb per wave at a time): 128-bit MACs with
+// the matrix row read through the scalar cache.
 __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
     k_bconv(const uint64_t *__restrict__ x, long in_stride, uint64_t *__restrict__ y, long out_stride,
             const uint64_t *__restrict__ mods, const uint64_t *__restrict__ mus,
@@ -58,16 +62,17 @@ __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
     const int k0 = (blockIdx.x % tiles) * BC_TILE;
     const uint64_t *xp = x + poly * in_stride;
     uint64_t *yp = y + poly * out_stride;
-    const int lane = threadIdx.x % BC_TILE, wave = threadIdx.x / BC_TILE;
+    const int lane = threadIdx.x % BC_TILE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / BC_TILE);
+    for (int i = wave; i < in_n; i += BC_WAVES) {
+        const uint64_t qi = mods[in0 + i];
+        const kbfv_tw h = hatinv[i];
+        ys[i][lane] = kb_shoup(xp[(long)i * n + k0 + lane], h.w, h.ws, qi);
+    }
+    __syncthreads();
     if (wave == 0) {
         double frac = 0.0;
-        for (int i = 0; i < in_n; ++i) {
-            const uint64_t qi = mods[in0 + i];
-            const kbfv_tw h = hatinv[i];
-            const uint64_t v = kb_shoup(xp[(long)i * n + k0 + lane], h.w, h.ws, qi);
-            ys[i][lane] = v;
-            frac = fma((double)v, inv[i], frac);
-        }
+        for (int i = 0; i < in_n; ++i) frac = fma((double)ys[i][lane], inv[i], frac);
         alpha_s[lane] = (int)llround(frac);
     }
     __syncthreads();
@@ -77,7 +82,14 @@ __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
         const uint64_t pj = mods[m];
         const uint64_t *row = hat_mod_out + (long)j * in_n;
         kbfv_u128 acc{0, 0};
-        for (int i = 0; i < in_n; ++i) kb_mac128(acc, ys[i][lane], row[i]);
+        int i = 0;
+        for (; i + 4 <= in_n; i += 4) {
+            kb_mac128(acc, ys[i][lane], row[i]);
+            kb_mac128(acc, ys[i + 1][lane], row[i + 1]);
+            kb_mac128(acc, ys[i + 2][lane], row[i + 2]);
+            kb_mac128(acc, ys[i + 3][lane], row[i + 3]);
+        }
+        for (; i < in_n; ++i) kb_mac128(acc, ys[i][lane], row[i]);
         const uint64_t v = kb_barrett128(acc, pj, mus[2 * m], mus[2 * m + 1]);
         yp[(long)j * n + k0 + lane] = kb_submod(v, alpha_mod_out[(long)alpha * out_n + j], pj);
     }
@@ -162,14 +174,13 @@ __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
     const int sn = qn + rn;
     const uint64_t *xp = x + (long)poly * sn * n;
     uint64_t *yp = y + (long)poly * rn * n;
-    const int lane = threadIdx.x % BC_TILE, wave = threadIdx.x / BC_TILE;
+    const int lane = threadIdx.x % BC_TILE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / BC_TILE);
+    for (int i = wave; i < qn; i += BC_WAVES) xs[i][lane] = xp[(long)i * n + k0 + lane];
+    __syncthreads();
     if (wave == 0) {
         double nu = 0.5;
-        for (int i = 0; i < qn; ++i) {
-            const uint64_t v = xp[(long)i * n + k0 + lane];
-            xs[i][lane] = v;
-            nu = fma((double)v, t.sr_frac[i], nu);
-        }
+        for (int i = 0; i < qn; ++i) nu = fma((double)xs[i][lane], t.sr_frac[i], nu);
         alpha_s[lane] = (uint64_t)nu;
     }
     __syncthreads();
@@ -179,7 +190,14 @@ __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
         const uint64_t rj = t.mod[m];
         const uint64_t *row = t.sr_modr + (long)j * (qn + 1);
         kbfv_u128 acc{0, 0};
-        for (int i = 0; i < qn; ++i) kb_mac128(acc, xs[i][lane], row[i]);
+        int i = 0;
+        for (; i + 4 <= qn; i += 4) {
+            kb_mac128(acc, xs[i][lane], row[i]);
+            kb_mac128(acc, xs[i + 1][lane], row[i + 1]);
+            kb_mac128(acc, xs[i + 2][lane], row[i + 2]);
+            kb_mac128(acc, xs[i + 3][lane], row[i + 3]);
+        }
+        for (; i < qn; ++i) kb_mac128(acc, xs[i][lane], row[i]);
         kb_mac128(acc, xp[(long)(qn + j) * n + k0 + lane], row[qn]);
         const uint64_t v = kb_barrett128(acc, rj, t.mu[2 * m], t.mu[2 * m + 1]);
         const uint64_t a = kb_barrett128(kbfv_u128{alpha, 0}, rj, t.mu[2 * m], t.mu[2 * m + 1]);

@@ -1,0 +1,23 @@
+"""The C++ drop-in API (include/kunpbfv/bfv.hpp) driven exactly as the
+reference's correct_mult.cu drives its own classes: expected m1*m2*m1 mod t in
+every slot.  N = 16384 uses the reference's logN=14 set (two special primes),
+N = 32768 the reference's logN=15 chain (with gamma = 2).  The SEAL-style
+N = 4096 set (72-bit Q) has room for one product only, so it checks m1*m2."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "kunpbfv_amd", "examples", "correct_mult")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,extra", [(4096, ["--no-const"]), (16384, []), (32768, [])])
+def test_correct_mult_driver(n, extra):
+    assert os.path.exists(EXE), "example driver not built (run __graft_entry__.build())"
+    r = subprocess.run([EXE, str(n)] + extra, capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "slots_wrong=0" in r.stdout
