@@ -24,6 +24,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes per k_ntt_pass launch of the roofline measurement at N=2^16,
+# 28 Q + 29 R primes, 3·128 polynomials: rocprofv3 FETCH_SIZE and WRITE_SIZE
+# runs over `bench.py --roofline-only`, each scaled by the factor the same
+# counter reads for a 4 GiB device copy (fetch 2.000, write 1.000); recorded
+# in profiles/ntt_qr_inverse_n65536_b128_counters.md.
+NTT_TRAFFIC_BYTES = {(16, 28, 128): 22_982_298_368}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -36,6 +43,8 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline kernel and a 4 GiB copy (for counter runs)")
     return ap.parse_args()
 
 
@@ -80,6 +89,17 @@ def main():
     torch.cuda.init()
     ctx = kb.Context.chain60(args.log_n, args.qn, 1, 4, 2, 65537)
     B, N = args.batch, ctx.n
+    stream = torch.cuda.current_stream()
+    if args.roofline_only:
+        # calibration: a plain device copy of 4 GiB (4 GiB read + 4 GiB written)
+        src = torch.empty(1 << 29, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        print(json.dumps({"roofline": kernel_roofline(ctx, B, stream), "copy_bytes": 1 << 32}), flush=True)
+        return
     sk = ctx.keygen_sk(bytes([1]) * 32)
     pk = ctx.keygen_pk(bytes([2]) * 32, sk)
     rlk = ctx.keygen_rlk(bytes([3]) * 32, sk)
@@ -97,7 +117,6 @@ def main():
     out = ctx.empty(*ct1.shape)
     ws_bytes = ctx.mult_workspace_bytes(B)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
-    stream = torch.cuda.current_stream()
 
     # ---- correctness: decrypt the two checkable products
     ctx.mult_relin(rlk, ct1, ct2, out=out, ws=ws)
@@ -212,7 +231,8 @@ def kernel_roofline(ctx, B, stream):
     achieved = bytes_per_launch / per_launch_s / 1e9
     return {"kernel": "k_ntt_pass (inverse, QR basis, whole batch)", "bound": "hbm",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": NTT_TRAFFIC_BYTES.get((ctx.log_n, ctx.qn, B)),
             "launch_ms": round(per_launch_s * 1e3, 3), "bytes_per_launch": bytes_per_launch}
 
 
