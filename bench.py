@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the N=2^15 HIP-graph measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the mixed-N serving measurement")
+    ap.add_argument("--serving-qps", type=float, default=3000.0)
+    ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernel and a 4 GiB copy (for counter runs)")
     return ap.parse_args()
@@ -171,8 +175,8 @@ def main():
         plain_ops = B / (plain_ms / 1e3)
         plain_exact = bool(torch.equal(out1, ref_out))
 
-    # ---- dominant kernel: the forward NTT over the QR basis as the pipeline runs it
-    # (inverse QR transform of the 3 tensor polynomials of every ciphertext)
+    # ---- dominant kernel: the NTT pass, measured on the pipeline's largest launch
+    # (inverse transform over the QR basis of the 3 tensor polynomials of every ciphertext)
     roof = kernel_roofline(ctx, B, stream)
 
     result = {
@@ -202,7 +206,87 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, ct1, ct2, tuple(rlk.shape))
+    del ws, out, ct1, ct2, rlk
+    if not args.no_graph:
+        result["graph_n32768_b64"] = graph_c3(stream)
+    if not args.no_serving:
+        result["serving"] = serving_mix(args.serving_qps, args.serving_seconds)
     print(json.dumps(result), flush=True)
+
+
+def graph_c3(stream, batch=64, reps=5):
+    """N=2^15, 13 + 1 x 60-bit primes, t=65537 (batching), dnum=13, batch 64:
+    the batched pipeline captured as one HIP graph, replayed, against the same
+    batch as one eager batched call and as 64 launches of batch 1."""
+    import numpy as np
+    import torch
+    import kunpbfv_amd as kb
+    ctx = kb.Context.chain60(15, 13, 1, 4, 2, 65537)
+    sk = ctx.keygen_sk(bytes([7]) * 32)
+    pk = ctx.keygen_pk(bytes([8]) * 32, sk)
+    rlk = ctx.keygen_rlk(bytes([9]) * 32, sk)
+    rng = np.random.default_rng(15)
+    ms = []
+    cts = []
+    for seed in (10, 11):
+        m = rng.integers(0, ctx.plain, size=(batch, ctx.n), dtype=np.uint64)
+        md = kb.to_device(m)
+        ctx.encode(md)
+        ms.append(m)
+        cts.append(ctx.encrypt(bytes([seed]) * 32, pk, md))
+    c1, c2 = cts
+    nb = ctx.mult_workspace_bytes(batch)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    out_e, out_g, out_p = (ctx.empty(*c1.shape) for _ in range(3))
+    graph = ctx.mult_relin_graph(rlk, c1, c2, out_g, ws)
+    ws1 = torch.empty(ctx.mult_workspace_bytes(1), dtype=torch.uint8, device="cuda")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def plain():
+        for i in range(batch):
+            ctx.mult_relin(rlk, c1[i:i + 1], c2[i:i + 1], out=out_p[i:i + 1], ws=ws1)
+
+    t_e = timed(lambda: ctx.mult_relin(rlk, c1, c2, out=out_e, ws=ws))
+    t_g = timed(lambda: graph.launch(stream))
+    t_p = timed(plain)
+    dec = ctx.decrypt(sk, out_g)
+    ctx.decode(dec)
+    want = (ms[0].astype(object) * ms[1].astype(object)) % ctx.plain
+    ok = np.array_equal(kb.to_host(dec).astype(object), want)
+    res = {"workload": "HomMult+relin, N=2^15, 13+1 x 60-bit primes, t=65537, dnum=13, batch 64",
+           "graph_ops_per_s": round(batch / (t_g / 1e3), 2),
+           "eager_batched_ops_per_s": round(batch / (t_e / 1e3), 2),
+           "plain_ops_per_s": round(batch / (t_p / 1e3), 2),
+           "graph_over_plain": round(t_p / t_g, 3),
+           "graph_ms": round(t_g, 3), "eager_ms": round(t_e, 3), "plain_ms": round(t_p, 3),
+           "bit_exact_graph_eager_plain": bool(torch.equal(out_g, out_e) and torch.equal(out_g, out_p)),
+           "decrypts_to_product": bool(ok)}
+    del graph
+    return res
+
+
+def serving_mix(qps, seconds, seed=11):
+    """Seeded Poisson arrivals over N = 2^13..2^16 (40% HomMult+relin, 30%
+    encryption, 20% RGSW external product, 10% addition): the continuous-
+    batching loop against the serial batch-1 loop on the same requests."""
+    from kunpbfv_amd import serving
+    rings = {ln: serving.Ring(ln, pool=16, max_batch=64) for ln in (13, 14, 15, 16)}
+    reqs = serving.poisson_requests(qps, seconds, sorted(rings), seed, pool=16)
+    batched = serving.serve(rings, reqs)
+    plain = serving.serial_plain(rings, serving.copy_requests(reqs))
+    return {"workload": f"Poisson {qps:g} req/s for {seconds:g} s, N=2^13..2^16 (2/6/13/28 + 1 x 60-bit), "
+                        "40% mult+relin / 30% encrypt / 20% RGSW ext. product / 10% add, seed "
+                        f"{seed}", "batched": batched, "serial_plain": plain}
 
 
 def kernel_roofline(ctx, B, stream):

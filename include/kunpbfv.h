@@ -196,6 +196,19 @@ int kbfv_mult_relin(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t
                     const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
                     size_t ws_bytes, void *stream);
 
+/* HIP-graph form of kbfv_mult_relin: the batched pipeline for fixed buffers is
+ * captured once and each later kbfv_graph_launch replays every kernel with a
+ * single submission.  No counterpart in the reference, which launches every
+ * kernel from the host on each call (include/KeySwitch_23.cuh:244-310).  The
+ * buffers given at creation are the ones every replay reads and writes; they
+ * must outlive the graph. */
+typedef struct kbfv_graph kbfv_graph;
+int kbfv_mult_relin_graph_create(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t *ct1,
+                                 const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
+                                 size_t ws_bytes, kbfv_graph **graph);
+int kbfv_graph_launch(kbfv_graph *graph, void *stream);
+void kbfv_graph_destroy(kbfv_graph *graph);
+
 /* Batch encoder (BFVContext::encode / decode, include/Encoder.cuh:23-113):
  * INTT / NTT over t when t = 1 mod 2N, identity (coefficient encoding) otherwise.
  * In place on [batch][N]. */

@@ -61,6 +61,9 @@ SIGNATURES = {
     "kbfv_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_mult_relin_graph_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, C.POINTER(_vp)]),
+    "kbfv_graph_launch": (C.c_int, [_vp, _vp]),
+    "kbfv_graph_destroy": (None, [_vp]),
     "kbfv_encode": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "kbfv_decode": (C.c_int, [_vp, _vp, C.c_int, _vp]),
     "kbfv_rgsw_rows": (C.c_int, [_vp, C.c_int]),
@@ -345,6 +348,10 @@ class Context:
                                   stream_ptr(stream)))
         return out
 
+    def mult_relin_graph(self, rlk, c1, c2, out, ws):
+        """Capture kbfv_mult_relin on these buffers as a HIP graph (MultGraph)."""
+        return MultGraph(self, rlk, c1, c2, out, ws)
+
     def encode(self, vals, stream=None):
         check(lib.kbfv_encode(self.h, ptr(vals), vals.shape[0], stream_ptr(stream)))
         return vals
@@ -373,6 +380,26 @@ class Context:
         check(lib.kbfv_external_product(self.h, ptr(rgsw), ptr(ct), ptr(out), batch, base_bits, ptr(ws),
                                         ws.numel(), stream_ptr(stream)))
         return out
+
+
+class MultGraph:
+    """A captured batched HomMult+relin; launch() replays it on a stream.  Holds
+    references to its buffers so they outlive the graph."""
+
+    def __init__(self, ctx, rlk, c1, c2, out, ws):
+        self._bufs = (ctx, rlk, c1, c2, out, ws)
+        h = _vp()
+        check(lib.kbfv_mult_relin_graph_create(ctx.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), c1.shape[0],
+                                               ptr(ws), ws.numel(), C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        check(lib.kbfv_graph_launch(self.h, stream_ptr(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.kbfv_graph_destroy(self.h)
+            self.h = None
 
 
 def xsalsa20(key, nonce, length, block0=0, stream=None):

@@ -14,6 +14,7 @@
 #include <mutex>
 
 #include "kbfv_internal.h"
+#include "kbfv_mfma.h"
 
 // ---------------------------------------------------------------- errors
 static thread_local char g_err[512] = "";
@@ -505,6 +506,26 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     keepf("q_inv", q_inv), keepf("r_inv", r_inv), keepf("sr_frac", sr_frac);
     keepf("dec_frac", dec_frac), keepf("dec_B_frac", dec_B_frac);
 
+    // int8-MFMA operand tables (transposed matrices, byte-reversed balanced digits)
+    auto mfma_table = [&](int K, int J, auto entry) {
+        const int KP = 8 * ((K + 7) / 8), JP = 16 * ((J + 15) / 16);
+        std::vector<uint64_t> v((size_t)JP * KP, 0);
+        for (int j = 0; j < J; j++)
+            for (int i = 0; i < K; i++) v[(size_t)j * KP + i] = kb_host_digits_reversed(entry(i, j));
+        return v;
+    };
+    const bool mx_ok = rn + 1 <= 32;  // every matrix of the MFMA path has K, J <= 32
+    std::vector<uint64_t> mx_qr, mx_rq, mx_sr;
+    if (mx_ok) {
+        mx_qr = mfma_table(qn + 1, rn, [&](int i, int j) {
+            return i < qn ? q_hat_mod_r[j * qn + i] : (R[j] - alphaQ[rn + j]) % R[j];
+        });
+        mx_rq = mfma_table(rn + 1, qn, [&](int i, int j) {
+            return i < rn ? r_hat_mod_q[j * rn + i] : (Q[j] - alphaR[qn + j]) % Q[j];
+        });
+        mx_sr = mfma_table(qn, rn, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; });
+    }
+
     // device upload: one blob for the small tables
     Blob blob;
     std::vector<uint64_t> tvec_mod_q(Q), tvec_plain(qn, plain);
@@ -524,6 +545,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     size_t o_th = blob.add(tws(t_hatinv, T)), o_thp = blob.add(t_hat_mod_pq), o_tmp = blob.add(t_mod_pq);
     size_t o_half = blob.add(halfT), o_phq = blob.add(p_hat_mod_q), o_phi = blob.add(tws(p_hatinv, P));
     size_t o_pinv = blob.add(tws(Pinv, Q)), o_pmq = blob.add(P_mod_q);
+    size_t o_mxqr = blob.add(mx_qr), o_mxrq = blob.add(mx_rq), o_mxsr = blob.add(mx_sr);
 
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipMalloc(&c->d_blob, blob.bytes.size());
@@ -557,6 +579,10 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     d.t_mod_pq = (const uint64_t *)at(o_tmp), d.halfT_mod = (const uint64_t *)at(o_half);
     d.p_hat_mod_q = (const uint64_t *)at(o_phq), d.p_hatinv = (const kbfv_tw *)at(o_phi);
     d.Pinv_mod_q = (const kbfv_tw *)at(o_pinv), d.P_mod_q = (const uint64_t *)at(o_pmq);
+    if (mx_ok) {
+        d.mx_qr = (const uint64_t *)at(o_mxqr), d.mx_rq = (const uint64_t *)at(o_mxrq);
+        d.mx_sr = (const uint64_t *)at(o_mxsr);
+    }
     *out = c;
     return KBFV_OK;
 }

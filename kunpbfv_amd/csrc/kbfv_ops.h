@@ -16,12 +16,23 @@ int kb_hps_decrypt(const kbfv_context *c, const uint64_t *x, uint64_t *m, int ba
 int kb_delta_m(const kbfv_context *c, const uint64_t *m, uint64_t *out, int batch, hipStream_t st);
 int kb_reduce_plain(const kbfv_context *c, uint64_t *v, long count, hipStream_t st);
 
+// rns_mfma.hip: the same base conversion and scale-and-round on the int8 matrix
+// cores; return 1 (nothing launched) when the bases are too large for them
+bool kb_rns_gemm_fits(int k_rows, int out_n);
+int kb_bconv_mfma(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stride, uint64_t *y,
+                  long out_stride, int polys, hipStream_t st);
+int kb_scale_round_mfma(const kbfv_context *c, const uint64_t *x, uint64_t *y, int polys, hipStream_t st);
+
 // ks.hip
 int kb_modup_qj_to_t(const kbfv_context *c, const uint64_t *x, long xs, uint64_t *out, int batch,
                      hipStream_t st);
 int kb_modup_pq_to_t(const kbfv_context *c, const uint64_t *x, uint64_t *out, int batch, hipStream_t st);
 int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t *key, uint64_t *out,
                      int batch, hipStream_t st);
+// ks_mfma.hip: the inner product on the int8 matrix cores (returns 1 when the
+// key shape does not fit it)
+int kb_inner_product_mfma(const kbfv_context *c, const uint64_t *cin, const uint64_t *key, uint64_t *out,
+                          int batch, hipStream_t st);
 int kb_modup_t_to_pq(const kbfv_context *c, const uint64_t *in, uint64_t *out, int batch, hipStream_t st);
 int kb_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
                long add_stride, int batch, hipStream_t st);

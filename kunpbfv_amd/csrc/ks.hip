@@ -284,6 +284,10 @@ int kb_modup_pq_to_t(const kbfv_context *c, const uint64_t *x, uint64_t *out, in
 int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t *key, uint64_t *out,
                      int batch, hipStream_t st) {
     KB_REQUIRE(c && cin && key && out && batch > 0, "bad inner-product arguments");
+    if (batch >= 16) {  // enough ciphertexts to fill the matrix-core column tiles
+        const int r = kb_inner_product_mfma(c, cin, key, out, batch, st);
+        if (r != 1) return r;
+    }
     const int D = c->dnum, B = c->ri_blocks, tn = c->tn, n = c->n;
     const unsigned grid = (unsigned)((n / 64) * tn * ((B + IP_WAVES - 1) / IP_WAVES));
 #define KB_IP(MD)                                                                                        \

@@ -293,6 +293,10 @@ int kb_bconv(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stri
              long out_stride, int polys, hipStream_t st) {
     KB_REQUIRE(c && x && y && polys > 0, "bad bconv arguments");
     KB_REQUIRE(c->rn <= BC_MAXIN, "too many limbs for base conversion");
+    if (c->t.mx_qr && kb_rns_gemm_fits((q_to_r ? c->qn : c->rn) + 1, q_to_r ? c->rn : c->qn)) {
+        const int r = kb_bconv_mfma(c, q_to_r, x, in_stride, y, out_stride, polys, st);
+        if (r != 1) return r;
+    }
     const int tiles = c->n / BC_TILE;
     const kb_dev_tables &t = c->t;
     if (q_to_r)
@@ -330,6 +334,10 @@ int kb_tensor(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, u
 int kb_scale_round(const kbfv_context *c, const uint64_t *x, uint64_t *y, int polys, hipStream_t st) {
     KB_REQUIRE(c && x && y && polys > 0, "bad scale-round arguments");
     KB_REQUIRE(c->qn <= BC_MAXIN, "too many limbs for scale-and-round");
+    if (c->t.mx_sr && kb_rns_gemm_fits(c->qn, c->rn)) {
+        const int r = kb_scale_round_mfma(c, x, y, polys, st);
+        if (r != 1) return r;
+    }
     const int tiles = c->n / BC_TILE;
     hipLaunchKernelGGL(k_scale_round, dim3(tiles * polys), dim3(BC_TILE * BC_WAVES), 0, st, c->t, x, y,
                        c->qn, c->rn, c->oR, c->n);

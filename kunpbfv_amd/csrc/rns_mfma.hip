@@ -1,0 +1,212 @@
+// HPS base conversion and scale-and-round as int8 MFMA products.
+//
+// Both are (coefficient x constant matrix) products with exact 128-bit sums
+// before one Barrett reduction per output (include/poly_arithmetic.cuh:1139-1178
+// base_convert_matmul_hps_unroll2_kernel, :1263-1317 scaleAndRound_HPS_QR_R).
+// The product runs transposed, out^T[j][p] = sum_i M^T[j][i] X^T[i][p], so the
+// matrix is the MFMA A operand (rows = output limbs j) and the coefficients the
+// B operand (columns = positions p); a result lane then holds 16 consecutive
+// positions of 4 output limbs and its stores are 128-byte runs.
+//
+// Base conversion Q -> R (and R -> Q):  y_i = [x_i (Q/q_i)^-1]_{q_i};
+//   alpha = llround(sum_i y_i / q_i)   (double, input order, explicit fma);
+//   out_j = [sum_i y_i [Q/q_i]_{r_j} + alpha [-Q]_{r_j}]_{r_j}
+// with alpha riding in the product as one more K row.
+// Scale-and-round QR -> R:  nu = 0.5 + sum_{i<qn} x_i frac_i (double);
+//   out_j = [sum_{i<qn} x_i M_ij + x_{qn+j} c_j + (uint64)nu]_{r_j}.
+// The double estimates are the reference's, in its order; every other step is
+// exact, so outputs equal the scalar kernels' bit for bit.
+#include "kbfv_internal.h"
+#include "kbfv_mfma.h"
+#include "kbfv_ops.h"
+
+namespace {
+
+constexpr int RG_MT = 2;                 // 16-position tiles per wave
+constexpr int RG_POS = 16 * RG_MT;       // positions per wave
+constexpr int RG_WAVES = 4;              // waves per workgroup
+constexpr int RG_BLOCK = RG_POS * RG_WAVES;
+
+// KC: 8-row chunks of K (K <= 8 KC);  NJ: 16-row tiles of output limbs.
+// SR = false: base conversion (Shoup-scaled inputs, alpha row at K index in_n);
+// SR = true: scale-and-round (raw Q inputs, R-limb diagonal term and nu added after).
+template <int KC, int NJ, bool SR>
+__global__ void __launch_bounds__(64 * RG_WAVES, 2)
+    k_rns_gemm(const uint64_t *__restrict__ x, long in_stride, uint64_t *__restrict__ y, long out_stride,
+               const uint64_t *__restrict__ mat, const uint64_t *__restrict__ mods_out,
+               const uint64_t *__restrict__ mus_out, const kbfv_tw *__restrict__ hatinv,
+               const uint64_t *__restrict__ mods_in, const double *__restrict__ est,
+               const uint64_t *__restrict__ sr_rows, int in_n, int out_n, int n) {
+    constexpr int KP = 8 * KC;
+    __shared__ uint64_t ys[RG_WAVES][KP][RG_POS];
+    __shared__ uint64_t alpha_s[RG_WAVES][RG_POS];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int p16 = lane & 15, g = lane >> 4;
+    const int tiles = n / RG_BLOCK;
+    const long poly = blockIdx.x / tiles;
+    const int n0 = (blockIdx.x % tiles) * RG_BLOCK + wave * RG_POS;
+    const uint64_t *xp = x + poly * in_stride + n0;
+    uint64_t *yp = y + poly * out_stride + n0;
+
+    // inputs: lane (position p16 of tile mt, group g) holds K rows 8c + 2g + e
+    uint64_t yv[RG_MT][KC][2];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int i = 8 * c + 2 * g + e;
+            kbfv_tw h{0, 0};
+            uint64_t qi = 0;
+            if (!SR && i < in_n) h = hatinv[i], qi = mods_in[i];
+#pragma unroll
+            for (int mt = 0; mt < RG_MT; ++mt) {
+                uint64_t v = 0;
+                if (i < in_n) {
+                    v = xp[(long)i * n + 16 * mt + p16];
+                    if (!SR) v = kb_shoup(v, h.w, h.ws, qi);
+                    ys[wave][i][16 * mt + p16] = v;
+                }
+                yv[mt][c][e] = v;
+            }
+        }
+    }
+    __syncthreads();
+    // the reference's double estimate, one position per lane, in input order
+    {
+        const int pp = lane & (RG_POS - 1);
+        double acc = SR ? 0.5 : 0.0;
+        for (int i = 0; i < in_n; ++i) acc = fma((double)ys[wave][i][pp], est[i], acc);
+        if (lane < RG_POS) alpha_s[wave][pp] = SR ? (uint64_t)acc : (uint64_t)(int)llround(acc);
+    }
+    __syncthreads();
+
+    kb_v4i bop[RG_MT][KC];
+#pragma unroll
+    for (int mt = 0; mt < RG_MT; ++mt) {
+        const uint64_t al = alpha_s[wave][16 * mt + p16];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            uint64_t d0 = yv[mt][c][0], d1 = yv[mt][c][1];
+            if (!SR) {
+                if (8 * c + 2 * g == in_n) d0 = al;
+                if (8 * c + 2 * g + 1 == in_n) d1 = al;
+            }
+            bop[mt][c] = kb_pack2(kb_digits(d0), kb_digits(d1));
+        }
+    }
+    // one 16-limb output tile at a time: its matrix rows (byte-reversed digits),
+    // 15 weight passes folded by Horner, then the reduction and the stores
+#pragma unroll 1
+    for (int nt = 0; nt < NJ; ++nt) {
+        uint64_t rm[KC][2];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const uint64_t *mr = mat + (long)(16 * nt + p16) * KP + 8 * c + 2 * g;
+            rm[c][0] = mr[0];
+            rm[c][1] = mr[1];
+        }
+        int64_t S[RG_MT][4];
+        uint64_t Hl[RG_MT][4], Hh[RG_MT][4];
+#pragma unroll
+        for (int mt = 0; mt < RG_MT; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) S[mt][i] = 0, Hl[mt][i] = 0, Hh[mt][i] = 0;
+#pragma unroll
+        for (int w = 14; w >= 0; --w) {
+            kb_v4i acc[RG_MT];
+#pragma unroll
+            for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const kb_v4i a = kb_pack2(kb_weight_op(rm[c][0], w), kb_weight_op(rm[c][1], w));
+#pragma unroll
+                for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_mfma_i8(a, bop[mt][c], acc[mt]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < RG_MT; ++mt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    S[mt][i] = (int64_t)((uint64_t)S[mt][i] << 8) + (int64_t)acc[mt][i];
+                    if (w == 12 || w == 6 || w == 0) {
+                        kb_horner48(Hl[mt][i], Hh[mt][i], S[mt][i]);
+                        S[mt][i] = 0;
+                    }
+                }
+                // keep each weight pass together (no hoisting of later passes' operands)
+                __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 16 * nt + 4 * g + i;
+            if (j >= out_n) continue;
+            const uint64_t pj = mods_out[j], mlo = mus_out[2 * j], mhi = mus_out[2 * j + 1];
+#pragma unroll
+            for (int mt = 0; mt < RG_MT; ++mt) {
+                kbfv_u128 v{Hl[mt][i], Hh[mt][i]};
+                if (SR) {
+                    kb_mac128(v, xp[(long)(in_n + j) * n + 16 * mt + p16], sr_rows[(long)j * (in_n + 1) + in_n]);
+                    kb_add128(v, alpha_s[wave][16 * mt + p16]);
+                }
+                yp[(long)j * n + 16 * mt + p16] = kb_barrett128(v, pj, mlo, mhi);
+            }
+        }
+    }
+}
+
+template <bool SR>
+int launch_rns_gemm(int kc, int nj, dim3 grid, hipStream_t st, const uint64_t *x, long in_stride,
+                    uint64_t *y, long out_stride, const uint64_t *mat, const uint64_t *mods_out,
+                    const uint64_t *mus_out, const kbfv_tw *hatinv, const uint64_t *mods_in,
+                    const double *est, const uint64_t *sr_rows, int in_n, int out_n, int n) {
+#define KB_RG(KC, NJ)                                                                                 \
+    if (kc == KC && nj == NJ) {                                                                        \
+        hipLaunchKernelGGL((k_rns_gemm<KC, NJ, SR>), grid, dim3(64 * RG_WAVES), 0, st, x, in_stride, y, \
+                           out_stride, mat, mods_out, mus_out, hatinv, mods_in, est, sr_rows, in_n,     \
+                           out_n, n);                                                                   \
+        return 0;                                                                                      \
+    }
+    KB_RG(1, 1)
+    KB_RG(2, 1)
+    KB_RG(3, 1)
+    KB_RG(4, 1)
+    KB_RG(1, 2)
+    KB_RG(2, 2)
+    KB_RG(3, 2)
+    KB_RG(4, 2)
+#undef KB_RG
+    return 1;
+}
+
+}  // namespace
+
+// K rows (including the alpha row of base conversion) and output limbs a matrix
+// of the MFMA path can hold; larger bases use the scalar kernels.
+bool kb_rns_gemm_fits(int k_rows, int out_n) { return k_rows <= 32 && out_n <= 32; }
+
+int kb_bconv_mfma(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stride, uint64_t *y,
+                  long out_stride, int polys, hipStream_t st) {
+    const int in_n = q_to_r ? c->qn : c->rn, out_n = q_to_r ? c->rn : c->qn;
+    const int kc = (in_n + 1 + 7) / 8, nj = (out_n + 15) / 16;
+    const kb_dev_tables &t = c->t;
+    const int in0 = q_to_r ? c->oQ : c->oR, out0 = q_to_r ? c->oR : c->oQ;
+    const dim3 grid((unsigned)((long)polys * (c->n / RG_BLOCK)));
+    if (launch_rns_gemm<false>(kc, nj, grid, st, x, in_stride, y, out_stride, q_to_r ? t.mx_qr : t.mx_rq,
+                               t.mod + out0, t.mu + 2 * out0, q_to_r ? t.q_hatinv : t.r_hatinv, t.mod + in0,
+                               q_to_r ? t.q_inv : t.r_inv, nullptr, in_n, out_n, c->n))
+        return 1;
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+int kb_scale_round_mfma(const kbfv_context *c, const uint64_t *x, uint64_t *y, int polys, hipStream_t st) {
+    const int kc = (c->qn + 7) / 8, nj = (c->rn + 15) / 16;
+    const kb_dev_tables &t = c->t;
+    const dim3 grid((unsigned)((long)polys * (c->n / RG_BLOCK)));
+    if (launch_rns_gemm<true>(kc, nj, grid, st, x, (long)(c->qn + c->rn) * c->n, y, (long)c->rn * c->n,
+                              t.mx_sr, t.mod + c->oR, t.mu + 2 * c->oR, nullptr, nullptr, t.sr_frac,
+                              t.sr_modr, c->qn, c->rn, c->n))
+        return 1;
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}

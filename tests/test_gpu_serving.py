@@ -1,0 +1,76 @@
+"""GPU: the continuous-batching request loop.  Every request served inside a
+mixed batch must equal the same request run alone (batch 1): bit-exact for
+HomMult+relin, external product and addition; encryptions must decrypt to their
+message.  A burst of requests must actually be batched, and the serial plain
+loop must give the same outputs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rings():
+    from kunpbfv_amd import serving
+    return {12: serving.Ring(12, qn=2, pool=6, max_batch=16), 13: serving.Ring(13, pool=6, max_batch=16)}
+
+
+def _check_alone(rings, reqs):
+    import torch
+    import kunpbfv_amd as kb
+    from kunpbfv_amd import serving
+    torch.cuda.synchronize()
+    for r in reqs:
+        ring = rings[r.log_n]
+        if r.op == "encrypt":
+            dec = ring.ctx.decrypt(ring.sk, r.out[None].contiguous())
+            ring.ctx.decode(dec)
+            assert np.array_equal(kb.to_host(dec)[0], ring.msgs[r.a]), (r.rid, r.op)
+            continue
+        alone = serving.copy_requests([r])
+        ring.run(r.op, alone, stream=torch.cuda.current_stream(), keep=True)
+        torch.cuda.synchronize()
+        assert torch.equal(alone[0].out, r.out), (r.rid, r.op, r.log_n)
+
+
+def test_poisson_mix_matches_alone(rings):
+    from kunpbfv_amd import serving
+    reqs = serving.poisson_requests(qps=3000, duration_s=0.04, log_ns=[12, 13], seed=3, pool=6)
+    assert len(reqs) > 40 and {r.op for r in reqs} == set(serving.OPS)
+    stats = serving.serve(rings, reqs, keep=True)
+    assert stats["requests"] == len(reqs) and stats["p99_ms"] >= stats["p50_ms"] > 0
+    _check_alone(rings, reqs)
+
+
+def test_burst_is_batched_and_serial_loop_agrees(rings):
+    import torch
+    from kunpbfv_amd import serving
+    reqs = serving.poisson_requests(qps=1e6, duration_s=1e-4, log_ns=[12, 13], seed=4, pool=6)
+    assert len(reqs) >= 60
+    stats = serving.serve(rings, reqs, keep=True)
+    assert stats["mean_batch"] > 2, stats
+    _check_alone(rings, reqs)
+    ser = serving.copy_requests(reqs)
+    sstats = serving.serial_plain(rings, ser, keep=True)
+    assert sstats["requests"] == len(reqs)
+    for a, b in zip(reqs, ser):
+        if a.op != "encrypt":
+            assert torch.equal(a.out, b.out), (a.rid, a.op)
+
+
+def test_external_product_by_x_shifts_the_message(rings):
+    import kunpbfv_amd as kb
+    from kunpbfv_amd import serving
+    ring = rings[13]
+    r = serving.Request(0, 13, "extprod", 0.0, 2, 0)
+    ring.run("extprod", [r], keep=True)
+    dec = ring.ctx.decrypt(ring.sk, r.out[None].contiguous())
+    got = kb.to_host(dec)[0].astype(np.int64)
+    m = ring.msgs[2].astype(np.int64)
+    # coefficient form of the encoded message, multiplied by X (negacyclic)
+    enc = kb.to_host(ring.pt[2:3].contiguous())[0].astype(np.int64)
+    want = np.empty_like(enc)
+    want[1:] = enc[:-1]
+    want[0] = -enc[-1]
+    assert np.array_equal(got % ring.ctx.plain, want % ring.ctx.plain)
+    assert m.shape == got.shape
