@@ -84,7 +84,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of "
-                                   "this build (the batch shards by ciphertext, see DESIGN.md)"}))
+                                   "this benchmark (the batch shards by ciphertext across GPUs)"}))
         sys.exit(2)
     import numpy as np
     import torch

@@ -86,3 +86,9 @@ __device__ __forceinline__ uint64_t kb_addmod(uint64_t a, uint64_t b, uint64_t q
 __device__ __forceinline__ uint64_t kb_submod(uint64_t a, uint64_t b, uint64_t q) {
     return kb_csub(a + q - b, q);
 }
+
+// double -> uint64 truncation saturating at 2^64 - 1 for non-negative x (the
+// conversion the reference's nvcc build performs for static_cast<uint64_t>)
+__device__ __forceinline__ uint64_t kb_f64_to_u64_sat(double x) {
+    return x >= 18446744073709551616.0 ? ~0ULL : (uint64_t)x;
+}

@@ -54,11 +54,18 @@ __device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uin
     b = v;
 }
 
-// One register round of LR levels starting at pass level lam0.
-template <int LR, int LOG_S, bool COLUMN, bool FWD, bool NINV>
+// One register round of LR levels starting at pass level lam0.  The round reads
+// its 16 values from the LDS tile or, when GIN, straight from global memory
+// (the rounds whose lanes cover adjacent columns / 16 adjacent elements, i.e.
+// 128-byte runs); it writes them back to the tile or, when GOUT, straight to
+// global memory with the final reduction of the last pass (FIN).
+template <int LR, int LOG_S, bool COLUMN, bool FWD, bool NINV, bool GIN = false, bool GOUT = false,
+          bool FIN = false>
 __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const kbfv_tw *tab,
                                           const int log_n, const int g0, const int grp0,
-                                          const int stride, const uint64_t q, const kbfv_tw ninv) {
+                                          const int stride, const uint64_t q, const kbfv_tw ninv,
+                                          const uint64_t *__restrict__ gin = nullptr,
+                                          uint64_t *__restrict__ gout = nullptr) {
     using G = Geo<LOG_S, COLUMN>;
     constexpr int S = G::S, C = G::C, RR = 1 << LR;
     constexpr int NG = C * (S >> LR);
@@ -80,7 +87,12 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
         const int pstep = COLUMN ? stride : 1;
         uint64_t x[RR];
 #pragma unroll
-        for (int i = 0; i < RR; ++i) x[i] = tile[G::addr(c, kbase + i * d)];
+        for (int i = 0; i < RR; ++i) {
+            if constexpr (GIN)
+                x[i] = gin[posc + (kbase + i * d) * pstep];
+            else
+                x[i] = tile[G::addr(c, kbase + i * d)];
+        }
         if constexpr (FWD) {
 #pragma unroll
             for (int j = 0; j < LR; ++j) {
@@ -114,7 +126,41 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
             }
         }
 #pragma unroll
-        for (int i = 0; i < RR; ++i) tile[G::addr(c, kbase + i * d)] = x[i];
+        for (int i = 0; i < RR; ++i) {
+            if constexpr (GOUT) {
+                uint64_t v = x[i];
+                if constexpr (FIN) {
+                    if constexpr (FWD) v = kb_csub(v, q2);
+                    v = kb_csub(v, q);
+                }
+                gout[posc + (kbase + i * d) * pstep] = v;
+            } else {
+                tile[G::addr(c, kbase + i * d)] = x[i];
+            }
+        }
+    }
+    if constexpr (!GOUT) __syncthreads();
+}
+
+// coalesced global -> LDS copy of the workgroup's tile
+template <int LOG_S, bool COLUMN>
+__device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restrict__ in, int grp0,
+                                         int stride) {
+    using G = Geo<LOG_S, COLUMN>;
+    constexpr int S = G::S, C = G::C, T = G::T;
+#pragma unroll
+    for (int j = 0; j < TILE / T; ++j) {
+        const int e = threadIdx.x + j * T;
+        int c, k;
+        if constexpr (COLUMN) {
+            c = e % C;
+            k = e / C;
+        } else {
+            c = e / S;
+            k = e % S;
+        }
+        const int pos = COLUMN ? (grp0 * C + c + k * stride) : ((grp0 * C + c) * S + k);
+        tile[G::addr(c, k)] = in[pos];
     }
     __syncthreads();
 }
@@ -147,64 +193,48 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     const int stride = n >> LOG_S;
     const int g0 = COLUMN ? 0 : (log_n - LOG_S);
 
-#pragma unroll
-    for (int j = 0; j < TILE / T; ++j) {
-        const int e = threadIdx.x + j * T;
-        int c, k;
-        if constexpr (COLUMN) {
-            c = e % C;
-            k = e / C;
-        } else {
-            c = e / S;
-            k = e % S;
-        }
-        const int pos = COLUMN ? (grp0 * C + c + k * stride) : ((grp0 * C + c) * S + k);
-        tile[G::addr(c, k)] = in[pos];
-    }
-    __syncthreads();
-
     constexpr int LR0 = LOG_S < LOG_R ? LOG_S : LOG_R;
-    constexpr int REM1 = LOG_S - LR0;
-    constexpr int LR1 = REM1 < LOG_R ? REM1 : LOG_R;
-    constexpr int LR2 = REM1 - LR1;
-    static_assert(LR2 <= LOG_R, "at most three rounds");
-    const kbfv_tw ninv = (!FWD && LAST) ? ninv_tab[m] : kbfv_tw{0, 0};
+    constexpr int LR1 = LOG_S - LR0;
+    static_assert(LR1 > 0 && LR1 <= LOG_R, "two register rounds per pass");
     constexpr bool NI = (!FWD && LAST);
+    const kbfv_tw ninv = NI ? ninv_tab[m] : kbfv_tw{0, 0};
+    // the forward pass's first register round (stride S/16) and the inverse pass's last
+    // touch 128-byte runs in both layouts; the stride-1 rounds of the row pass
+    // go through the LDS tile for both their global load and store.
     if constexpr (FWD) {
-        ntt_round<LR0, LOG_S, COLUMN, true, false>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv);
-        if constexpr (LR1 > 0)
+        ntt_round<LR0, LOG_S, COLUMN, true, false, true>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv, in);
+        if constexpr (COLUMN) {
+            ntt_round<LR1, LOG_S, COLUMN, true, false, false, true, LAST>(tile, LR0, tab, log_n, g0, grp0,
+                                                                          stride, q, ninv, nullptr, out);
+            return;
+        } else {
             ntt_round<LR1, LOG_S, COLUMN, true, false>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
-        if constexpr (LR2 > 0)
-            ntt_round<LR2, LOG_S, COLUMN, true, false>(tile, LR0 + LR1, tab, log_n, g0, grp0, stride, q,
-                                                       ninv);
+        }
     } else {
-        if constexpr (LR2 > 0)
-            ntt_round<LR2, LOG_S, COLUMN, false, NI>(tile, LR0 + LR1, tab, log_n, g0, grp0, stride, q,
-                                                     ninv);
-        if constexpr (LR1 > 0)
+        if constexpr (COLUMN) {
+            ntt_round<LR1, LOG_S, COLUMN, false, NI, true>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv,
+                                                           in);
+        } else {
+            stage_in<LOG_S, COLUMN>(tile, in, grp0, stride);
             ntt_round<LR1, LOG_S, COLUMN, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
-        ntt_round<LR0, LOG_S, COLUMN, false, NI>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv);
+        }
+        ntt_round<LR0, LOG_S, COLUMN, false, NI, false, true, LAST>(tile, 0, tab, log_n, g0, grp0, stride, q,
+                                                                    ninv, nullptr, out);
+        return;
     }
 
+    // forward row pass: stride-1 round done in the tile, coalesced store from it
     const uint64_t q2 = q << 1;
 #pragma unroll
     for (int j = 0; j < TILE / T; ++j) {
         const int e = threadIdx.x + j * T;
-        int c, k;
-        if constexpr (COLUMN) {
-            c = e % C;
-            k = e / C;
-        } else {
-            c = e / S;
-            k = e % S;
-        }
-        const int pos = COLUMN ? (grp0 * C + c + k * stride) : ((grp0 * C + c) * S + k);
+        const int c = e / S, k = e % S;
         uint64_t v = tile[G::addr(c, k)];
         if constexpr (LAST) {
-            if constexpr (FWD) v = kb_csub(v, q2);
+            v = kb_csub(v, q2);
             v = kb_csub(v, q);
         }
-        out[pos] = v;
+        out[(grp0 * C + c) * S + k] = v;
     }
 }
 

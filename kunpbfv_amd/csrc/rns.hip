@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
     if (wave == 0) {
         double nu = 0.5;
         for (int i = 0; i < qn; ++i) nu = fma((double)xs[i][lane], t.sr_frac[i], nu);
-        alpha_s[lane] = (uint64_t)nu;
+        alpha_s[lane] = kb_f64_to_u64_sat(nu);
     }
     __syncthreads();
     const uint64_t alpha = alpha_s[lane];

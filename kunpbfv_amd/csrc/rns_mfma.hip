@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
         const int pp = lane & (RG_POS - 1);
         double acc = SR ? 0.5 : 0.0;
         for (int i = 0; i < in_n; ++i) acc = fma((double)ys[wave][i][pp], est[i], acc);
-        if (lane < RG_POS) alpha_s[wave][pp] = SR ? (uint64_t)acc : (uint64_t)(int)llround(acc);
+        if (lane < RG_POS) alpha_s[wave][pp] = SR ? kb_f64_to_u64_sat(acc) : (uint64_t)(int)llround(acc);
     }
     __syncthreads();
 

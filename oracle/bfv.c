@@ -7,12 +7,15 @@
  *   SecretKey ctor (NTT over PQ)              include/SecretKey.cuh:20-37
  *   BFVScheme::addEncKey                      include/BFVScheme.cuh:57-85
  *   BFVScheme::addMultKey_23                  include/KeySwitch_23.cuh:21-66
+ *   BFVScheme::addLeftRotKey_23               include/KeySwitch_23.cuh:120-166
+ *   BFVScheme::leftRotateAndEqual_23          include/KeySwitch_23.cuh:544-578
+ *   sk_and_poly_LeftRot_kernel                include/poly_arithmetic.cuh:496-512
  *   BFVScheme::encryptZero / encryptMsg       include/BFVScheme.cuh:87-146
  *   BFVScheme::decryptMsg                     include/BFVScheme.cuh:148-166
  *   compute_t_QInv_c0c1c2 (HPS tensoring)     include/KeySwitch_23.cuh:703-926
  *   multAndEqual_23 relinearisation steps     include/KeySwitch_23.cuh:270-305
  *   BFVContext::encode / decode               include/Encoder.cuh:23-113
- * Deviations (all of them documented in DESIGN.md): the secret-key shortcut
+ * Deviations from the reference: the secret-key shortcut
  * that overwrites the product (include/KeySwitch_23.cuh:308, :823-832) is not
  * reproduced -- the result here is the relinearised tensor product; the NTT of
  * the Q part of the QR-extended inputs is taken from the NTT-domain input
@@ -89,43 +92,117 @@ void or_keygen_pk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq
 }
 
 /* relinearisation key in T: [2][dnum][ri_blocks][tn][N] (NTT over T) */
-void or_keygen_rlk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq,
-                   uint64_t *rlk_t) {
+/* Key-switching key from keypoly (over P u Q, NTT) to the secret, for the
+ * source polynomial src (over Q, NTT): digit d holds
+ *   a_d uniform, b_d = -a_d * keypoly + e_d + P * src on the limbs of digit d,
+ * converted to T (modUpPQtoT) and NTT over T; layout [2][dnum][ri_blocks][tn][N]
+ * (include/KeySwitch_23.cuh:21-66 with src = s^2, keypoly = s). */
+static void keygen_ksk(const or_ctx *c, const uint8_t seed[32], uint64_t tag_a, uint64_t tag_e,
+                       const uint64_t *src_q, const uint64_t *keypoly_pq, uint64_t *key_t) {
     size_t N = c->n;
     int pn = c->pn, qn = c->qn, pq = pn + qn, D = c->dnum, B = c->ri_blocks, tn = c->tn;
     size_t PQ = (size_t)pq * N;
-    uint64_t *s2 = alloc0((size_t)qn * N), *tmp = alloc0(PQ), *e = alloc0(PQ);
+    uint64_t *tmp = alloc0(PQ), *e = alloc0(PQ);
     uint64_t *a = alloc0(PQ), *b = alloc0(PQ), *conv = alloc0((size_t)B * tn * N);
-    pw_mul(c, sk_pq + (size_t)pn * N, sk_pq + (size_t)pn * N, s2, c->oQ, qn);
     size_t keyhalf = (size_t)D * B * tn * N;
     for (int d = 0; d < D; d++) {
         memset(tmp, 0, PQ * sizeof(uint64_t));
-        /* P * s^2 on the limbs of digit d (include/poly_arithmetic.cuh:469) */
+        /* P * src on the limbs of digit d (include/poly_arithmetic.cuh:469) */
         for (int j = d * pn; j < d * pn + pn && j < qn; j++) {
             uint64_t q = c->mod[c->oQ + j];
             for (size_t k = 0; k < N; k++)
-                tmp[(size_t)(pn + j) * N + k] = or_mulmod(s2[(size_t)j * N + k], c->P_mod_q[j], q);
+                tmp[(size_t)(pn + j) * N + k] = or_mulmod(src_q[(size_t)j * N + k], c->P_mod_q[j], q);
         }
-        or_sample_cbd(c, seed, OR_TAG(OR_TAG_RLK_E, 0), d, c->oP, pq, e);
+        or_sample_cbd(c, seed, tag_e, d, c->oP, pq, e);
         or_ntt_fwd_range(c, e, c->oP, pq, 1);
         pw_add(c, tmp, e, tmp, c->oP, pq);
-        or_sample_uniform(c, seed, OR_TAG(OR_TAG_RLK_A, 0), d, c->oP, pq, a);
-        pw_mul(c, a, sk_pq, b, c->oP, pq);
+        or_sample_uniform(c, seed, tag_a, d, c->oP, pq, a);
+        pw_mul(c, a, keypoly_pq, b, c->oP, pq);
         pw_sub(c, tmp, b, b, c->oP, pq);
         or_ntt_inv_range(c, a, c->oP, pq, 1);
         or_ntt_inv_range(c, b, c->oP, pq, 1);
         for (int h = 0; h < 2; h++) {
             or_modup_pq_to_t(c, h == 0 ? a : b, conv);
             or_ntt_fwd_range(c, conv, c->oT, tn, B);
-            memcpy(rlk_t + h * keyhalf + (size_t)d * B * tn * N, conv, (size_t)B * tn * N * 8);
+            memcpy(key_t + h * keyhalf + (size_t)d * B * tn * N, conv, (size_t)B * tn * N * 8);
         }
     }
-    free(s2);
     free(tmp);
     free(e);
     free(a);
     free(b);
     free(conv);
+}
+
+void or_keygen_rlk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq,
+                   uint64_t *rlk_t) {
+    size_t N = c->n;
+    int pn = c->pn, qn = c->qn;
+    uint64_t *s2 = alloc0((size_t)qn * N);
+    pw_mul(c, sk_pq + (size_t)pn * N, sk_pq + (size_t)pn * N, s2, c->oQ, qn);
+    keygen_ksk(c, seed, OR_TAG(OR_TAG_RLK_A, 0), OR_TAG(OR_TAG_RLK_E, 0), s2, sk_pq, rlk_t);
+    free(s2);
+}
+
+/* ---------------- Galois automorphisms and rotation ---------------- */
+uint64_t or_galois_elt(const or_ctx *c, long steps) {
+    uint64_t m2 = 2 * (uint64_t)c->n;
+    if (steps < 0) return m2 - 1;
+    uint64_t g = 1;
+    for (long i = 0; i < steps; i++) g = g * 5 % m2;
+    return g;
+}
+
+void or_automorphism(const or_ctx *c, uint64_t g, const uint64_t *in, uint64_t *out, int mod0,
+                     int limbs, int polys) {
+    size_t N = c->n;
+    uint64_t m2 = 2 * (uint64_t)N;
+    for (int y = 0; y < polys * limbs; y++) {
+        uint64_t q = c->mod[mod0 + y % limbs];
+        const uint64_t *src = in + (size_t)y * N;
+        uint64_t *dst = out + (size_t)y * N;
+        for (size_t i = 0; i < N; i++) {
+            uint64_t j = (uint64_t)i * g % m2;
+            if (j < N)
+                dst[j] = src[i];
+            else
+                dst[j - N] = src[i] ? q - src[i] : 0;
+        }
+    }
+}
+
+static uint64_t inv_mod_2n(uint64_t g, uint64_t m2) {
+    for (uint64_t x = 1; x < m2; x += 2)
+        if (g * x % m2 == 1) return x;
+    return 0;
+}
+
+void or_keygen_galois(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq, uint64_t g,
+                      uint64_t *key_t) {
+    size_t N = c->n;
+    int pn = c->pn, pq = pn + c->qn;
+    uint64_t *sc = alloc0((size_t)pq * N), *sr = alloc0((size_t)pq * N);
+    memcpy(sc, sk_pq, (size_t)pq * N * 8);
+    or_ntt_inv_range(c, sc, c->oP, pq, 1);
+    or_automorphism(c, inv_mod_2n(g, 2 * N), sc, sr, c->oP, pq, 1);
+    or_ntt_fwd_range(c, sr, c->oP, pq, 1);
+    keygen_ksk(c, seed, OR_TAG(OR_TAG_GAL_A, g), OR_TAG(OR_TAG_GAL_E, g), sk_pq + (size_t)pn * N, sr,
+               key_t);
+    free(sc);
+    free(sr);
+}
+
+void or_rotate(const or_ctx *c, const uint64_t *key_t, uint64_t g, const uint64_t *ct, uint64_t *out) {
+    size_t N = c->n, L = (size_t)c->qn * N;
+    uint64_t *x = alloc0(2 * L), *ks = alloc0(2 * L);
+    memcpy(x, ct, 2 * L * 8);
+    or_ntt_inv_range(c, x, c->oQ, c->qn, 2);
+    or_relin(c, key_t, x, ks);                              /* KS of a (coefficient form) */
+    pw_add(c, ks + L, x + L, ks + L, c->oQ, c->qn);         /* + b */
+    or_automorphism(c, g, ks, out, c->oQ, c->qn, 2);
+    or_ntt_fwd_range(c, out, c->oQ, c->qn, 2);
+    free(x);
+    free(ks);
 }
 
 /* ct = [a | b] over Q, NTT form.  m: [N] plaintext coefficients (already encoded) */

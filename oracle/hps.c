@@ -94,7 +94,9 @@ void or_scale_round(const or_ctx *c, const uint64_t *x_qr, uint64_t *y_r) {
     for (size_t k = 0; k < N; k++) {
         double nu = 0.5;
         for (int i = 0; i < qn; i++) nu = fma((double)x_qr[(size_t)i * N + k], c->sr_frac[i], nu);
-        uint64_t alpha = (uint64_t)nu;
+        /* the reference's static_cast<uint64> of nu is a saturating conversion
+         * (nvcc's cvt.rzi.u64.f64): nu can exceed 2^64 when every x_i is near q_i */
+        uint64_t alpha = nu >= 18446744073709551616.0 ? UINT64_MAX : (uint64_t)nu;
         for (int j = 0; j < rn; j++) {
             uint64_t rj = c->mod[c->oR + j];
             const uint64_t *row = c->sr_modr + (size_t)j * (qn + 1);

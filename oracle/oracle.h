@@ -114,7 +114,9 @@ enum {
     OR_TAG_ENC_E0 = 7,
     OR_TAG_ENC_E1 = 8,
     OR_TAG_RGSW_A = 9,
-    OR_TAG_RGSW_E = 10
+    OR_TAG_RGSW_E = 10,
+    OR_TAG_GAL_A = 11,
+    OR_TAG_GAL_E = 12
 };
 
 /* ---------------- BFV scheme (bfv.c) ---------------- */
@@ -133,6 +135,22 @@ void or_add(const or_ctx *c, const uint64_t *a, const uint64_t *b, uint64_t *out
             int mod0, int poly_num);
 void or_encode(const or_ctx *c, const uint64_t *slots, uint64_t *m);
 void or_decode(const or_ctx *c, const uint64_t *m, uint64_t *slots);
+
+/* Galois automorphisms X -> X^g (g odd, mod 2N) and slot rotation.
+ * or_galois_elt: 5^steps mod 2N (the reference's rotGroups, include/BFVcontext.cuh:788-796);
+ * steps < 0 gives the conjugation element 2N - 1.
+ * or_automorphism: coefficient form, polys x limbs starting at mod0, canonical output
+ *   (include/poly_arithmetic.cuh:496-512 sk_and_poly_LeftRot_kernel).
+ * or_keygen_galois: key-switching key in T, layout of the relinearisation key, that
+ *   switches from sigma_g^-1(s) to s (include/KeySwitch_23.cuh:120-166).
+ * or_rotate: ct (NTT over Q) -> sigma_g(KS(a) + (0, b)), NTT over Q
+ *   (include/KeySwitch_23.cuh:544-578). */
+uint64_t or_galois_elt(const or_ctx *c, long steps);
+void or_automorphism(const or_ctx *c, uint64_t g, const uint64_t *in, uint64_t *out, int mod0,
+                     int limbs, int polys);
+void or_keygen_galois(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq, uint64_t g,
+                      uint64_t *key_t);
+void or_rotate(const or_ctx *c, const uint64_t *key_t, uint64_t g, const uint64_t *ct, uint64_t *out);
 
 /* RGSW x RLWE external product, gadget base 2^base_bits over each Q limb */
 int or_rgsw_rows(const or_ctx *c, int base_bits);

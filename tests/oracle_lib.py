@@ -78,6 +78,10 @@ def _load():
         "or_rgsw_rows": (C.c_int, [C.c_void_p, C.c_int]),
         "or_rgsw_encrypt": (None, [C.c_void_p, u8p, u64p, u64p, C.c_int, u64p]),
         "or_external_product": (None, [C.c_void_p, u64p, u64p, C.c_int, u64p]),
+        "or_galois_elt": (C.c_uint64, [C.c_void_p, C.c_long]),
+        "or_automorphism": (None, [C.c_void_p, C.c_uint64, u64p, u64p, C.c_int, C.c_int, C.c_int]),
+        "or_keygen_galois": (None, [C.c_void_p, u8p, u64p, C.c_uint64, u64p]),
+        "or_rotate": (None, [C.c_void_p, u64p, C.c_uint64, u64p, u64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -266,6 +270,29 @@ class Ctx:
         rg, c = u64(rgsw), u64(ct)
         out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
         lib.or_external_product(self.h, P(rg), P(c), base_bits, P(out))
+        return out
+
+    # ---- Galois automorphisms / rotation ----
+    def galois_elt(self, steps):
+        return int(lib.or_galois_elt(self.h, steps))
+
+    def automorphism(self, g, a, mod0, limbs, polys=1):
+        a = u64(a)
+        out = np.zeros_like(a)
+        lib.or_automorphism(self.h, g, P(a), P(out), mod0, limbs, polys)
+        return out
+
+    def keygen_galois(self, seed, sk, g):
+        s, sp = B(seed)
+        sk = u64(sk)
+        key = np.zeros((2, self.dnum, self.ri_blocks, self.tn, self.n), dtype=np.uint64)
+        lib.or_keygen_galois(self.h, sp, P(sk), g, P(key))
+        return key
+
+    def rotate(self, key, g, ct):
+        k, c = u64(key), u64(ct)
+        out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
+        lib.or_rotate(self.h, P(k), g, P(c), P(out))
         return out
 
 
