@@ -215,6 +215,30 @@ void kbfv_graph_destroy(kbfv_graph *graph);
 int kbfv_encode(const kbfv_context *ctx, uint64_t *vals, int batch, void *stream);
 int kbfv_decode(const kbfv_context *ctx, uint64_t *vals, int batch, void *stream);
 
+/* ---------------------------------------------------------------- Galois automorphisms */
+
+/* Galois element of a left rotation by `steps`: 5^steps mod 2N (the reference's
+ * rotGroups, include/BFVcontext.cuh:788-796); steps < 0 gives the conjugation
+ * element 2N - 1.  Returns 0 for a null context. */
+uint64_t kbfv_galois_elt(const kbfv_context *ctx, int64_t steps);
+/* Coefficient-form automorphism X -> X^g on [polys][limbs][N] limbs from mod0
+ * (sk_and_poly_LeftRot_kernel, include/poly_arithmetic.cuh:496-512; outputs
+ * canonical).  in and out must differ. */
+int kbfv_automorphism(const kbfv_context *ctx, uint64_t g, const uint64_t *in, uint64_t *out, int mod0,
+                      int limbs, int polys, void *stream);
+/* Galois key for g (BFVScheme::addLeftRotKey_23 / addConjKey_23,
+ * include/KeySwitch_23.cuh:70-166): switches from sigma_g^-1(s) to s, in the
+ * relinearisation key's layout [2][dnum][ri_blocks][tn][N], NTT over T. */
+size_t kbfv_keygen_galois_workspace_bytes(const kbfv_context *ctx);
+int kbfv_keygen_galois(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *sk, uint64_t g,
+                       uint64_t *key, void *workspace, size_t ws_bytes, void *stream);
+/* Rotation / conjugation by g (BFVScheme::leftRotateAndEqual_23, conjugate_23,
+ * include/KeySwitch_23.cuh:466-578): out = sigma_g(KS(a) + (0, b)), batched.
+ * ct, out: [batch][2][qn][N] NTT form; out may alias ct. */
+size_t kbfv_rotate_workspace_bytes(const kbfv_context *ctx, int batch);
+int kbfv_rotate(const kbfv_context *ctx, const uint64_t *gkey, uint64_t g, const uint64_t *ct, uint64_t *out,
+                int batch, void *workspace, size_t ws_bytes, void *stream);
+
 /* ---------------------------------------------------------------- RGSW x RLWE */
 
 /* No counterpart in the reference (its "external product" is the key-switch

@@ -21,6 +21,7 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <map>
 #include <vector>
 
 #include "../kunpbfv.h"
@@ -138,8 +139,9 @@ class BFVScheme {
 
     void addEncKey(SecretKey &secretKey, hipStream_t stream = 0);
     void addMultKey_23(SecretKey &secretKey, hipStream_t stream = 0);
-    // rotation keys are outside this engine's scope: accepted and ignored
+    // Galois keys: left rotation by rot (g = 5^rot mod 2N) and conjugation (g = 2N - 1)
     void addLeftRotKey_23(SecretKey &secretKey, long rot, hipStream_t stream = 0);
+    void addConjKey_23(SecretKey &secretKey, hipStream_t stream = 0);
 
     void encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t stream = 0);
     void decryptMsg(Plaintext &plain, SecretKey &secretKey, Ciphertext &cipher, hipStream_t stream = 0);
@@ -156,12 +158,22 @@ class BFVScheme {
     // plaintext (from encode_ntt) times ciphertext (include/BFVScheme.cuh:226)
     void multConstAndEqual(Ciphertext &cipher, Plaintext &cnst);
 
+    // automorphisms with key switching (include/KeySwitch_23.cuh:466-584)
+    void leftRotateAndEqual_23(Ciphertext &cipher, long rotSlots);
+    void rightRotateAndEqual_23(Ciphertext &cipher, long rotSlots);  // left by N/2 - 2^rotSlots
+    void conjugate_23(Ciphertext &cipher_res, Ciphertext &cipher);
+    void conjugateAndEqual_23(Ciphertext &cipher);
+
     BFVContext &context;
     Key *publicKey = nullptr;
     Key_decomp *rlk_23 = nullptr;
+    std::map<long, Key_decomp *> rotKey_vec_23;  // by rotation amount
+    Key_decomp *ConjKey_23 = nullptr;
 
    private:
     void seed_next(uint8_t out[32]);
+    Key_decomp *galois_key(SecretKey &sk, uint64_t g, hipStream_t stream);
+    void apply_galois(Ciphertext &res, Ciphertext &cipher, Key_decomp *key, uint64_t g);
     void *ensure_ws(size_t bytes);
     uint64_t seed_state = 0x6b756e7042465600ULL;
     void *ws = nullptr;

@@ -61,6 +61,12 @@ SIGNATURES = {
     "kbfv_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_galois_elt": (_u64, [_vp, _i64]),
+    "kbfv_automorphism": (C.c_int, [_vp, _u64, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "kbfv_keygen_galois_workspace_bytes": (_sz, [_vp]),
+    "kbfv_keygen_galois": (C.c_int, [_vp, _vp, _vp, _u64, _vp, _vp, _sz, _vp]),
+    "kbfv_rotate_workspace_bytes": (_sz, [_vp, C.c_int]),
+    "kbfv_rotate": (C.c_int, [_vp, _vp, _u64, _vp, _vp, C.c_int, _vp, _sz, _vp]),
     "kbfv_mult_relin_graph_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, C.POINTER(_vp)]),
     "kbfv_graph_launch": (C.c_int, [_vp, _vp]),
     "kbfv_graph_destroy": (None, [_vp]),
@@ -346,6 +352,29 @@ class Context:
         ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
         check(lib.kbfv_mult_relin(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
                                   stream_ptr(stream)))
+        return out
+
+    # ---- Galois automorphisms / rotation
+    def galois_elt(self, steps):
+        return int(lib.kbfv_galois_elt(self.h, steps))
+
+    def automorphism(self, g, x, out, mod0, limbs, polys=1, stream=None):
+        check(lib.kbfv_automorphism(self.h, g, ptr(x), ptr(out), mod0, limbs, polys, stream_ptr(stream)))
+        return out
+
+    def keygen_galois(self, seed, sk, g, stream=None):
+        key = self.empty(2, self.dnum, self.ri_blocks, self.tn, self.n)
+        ws = self.workspace(lib.kbfv_keygen_galois_workspace_bytes(self.h))
+        check(lib.kbfv_keygen_galois(self.h, _seed(seed), ptr(sk), g, ptr(key), ptr(ws), ws.numel(),
+                                     stream_ptr(stream)))
+        return key
+
+    def rotate(self, gkey, g, ct, out=None, stream=None):
+        batch = ct.shape[0]
+        out = out if out is not None else self.empty(*ct.shape)
+        ws = self.workspace(lib.kbfv_rotate_workspace_bytes(self.h, batch))
+        check(lib.kbfv_rotate(self.h, ptr(gkey), g, ptr(ct), ptr(out), batch, ptr(ws), ws.numel(),
+                              stream_ptr(stream)))
         return out
 
     def mult_relin_graph(self, rlk, c1, c2, out, ws):

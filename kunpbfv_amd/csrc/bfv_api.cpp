@@ -217,6 +217,8 @@ BFVScheme::BFVScheme(BFVContext &ctx) : context(ctx) {}
 BFVScheme::~BFVScheme() {
     delete publicKey;
     delete rlk_23;
+    for (auto &kv : rotKey_vec_23) delete kv.second;
+    delete ConjKey_23;
     dfree(ws);
 }
 
@@ -269,13 +271,55 @@ void BFVScheme::addMultKey_23(SecretKey &sk, hipStream_t stream) {
     hip_ok(hipStreamSynchronize(stream), "relinearisation key");
 }
 
-void BFVScheme::addLeftRotKey_23(SecretKey &, long, hipStream_t) {
-    static bool warned = false;
-    if (!warned) {
-        fprintf(stderr, "kunpbfv: rotation keys are not supported; addLeftRotKey_23 ignored\n");
-        warned = true;
-    }
+Key_decomp *BFVScheme::galois_key(SecretKey &sk, uint64_t g, hipStream_t stream) {
+    auto *key = new Key_decomp((int)context.N, context.dnum, context.t_num, context.Ri_blockNum);
+    uint8_t seed[32];
+    seed_next(seed);
+    void *w = ensure_ws(kbfv_keygen_galois_workspace_bytes(context.handle));
+    ok(kbfv_keygen_galois(context.handle, seed, (uint64_t *)sk.sx_device, g, (uint64_t *)key->cipher_device, w,
+                          ws_bytes, stream),
+       "Galois key");
+    hip_ok(hipStreamSynchronize(stream), "Galois key");
+    return key;
 }
+
+void BFVScheme::addLeftRotKey_23(SecretKey &sk, long rot, hipStream_t stream) {
+    if (rotKey_vec_23.count(rot)) return;  // as the reference: generated once
+    rotKey_vec_23[rot] = galois_key(sk, kbfv_galois_elt(context.handle, rot), stream);
+}
+
+void BFVScheme::addConjKey_23(SecretKey &sk, hipStream_t stream) {
+    if (ConjKey_23) return;
+    ConjKey_23 = galois_key(sk, kbfv_galois_elt(context.handle, -1), stream);
+}
+
+void BFVScheme::apply_galois(Ciphertext &res, Ciphertext &cipher, Key_decomp *key, uint64_t g) {
+    if (cipher.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    res.l = cipher.l;
+    void *w = ensure_ws(kbfv_rotate_workspace_bytes(context.handle, 1));
+    ok(kbfv_rotate(context.handle, (uint64_t *)key->cipher_device, g, (uint64_t *)cipher.cipher_device,
+                   (uint64_t *)res.cipher_device, 1, w, ws_bytes, context.stream),
+       "rotate");
+    hip_ok(hipStreamSynchronize(context.stream), "rotate");
+}
+
+void BFVScheme::leftRotateAndEqual_23(Ciphertext &cipher, long rotSlots) {
+    auto it = rotKey_vec_23.find(rotSlots);
+    if (it == rotKey_vec_23.end()) throw std::invalid_argument("rotKey_23 not exists");
+    apply_galois(cipher, cipher, it->second, kbfv_galois_elt(context.handle, rotSlots));
+}
+
+void BFVScheme::rightRotateAndEqual_23(Ciphertext &cipher, long rotSlots) {
+    // include/KeySwitch_23.cuh:580-584
+    leftRotateAndEqual_23(cipher, (long)(context.N / 2) - (1L << rotSlots));
+}
+
+void BFVScheme::conjugate_23(Ciphertext &cipher_res, Ciphertext &cipher) {
+    if (!ConjKey_23) throw std::invalid_argument("conjKey_23 not exists");
+    apply_galois(cipher_res, cipher, ConjKey_23, kbfv_galois_elt(context.handle, -1));
+}
+
+void BFVScheme::conjugateAndEqual_23(Ciphertext &cipher) { conjugate_23(cipher, cipher); }
 
 void BFVScheme::encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t stream) {
     if (!publicKey) throw std::invalid_argument("public key not generated (call addEncKey)");

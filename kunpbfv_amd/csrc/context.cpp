@@ -506,24 +506,41 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     keepf("q_inv", q_inv), keepf("r_inv", r_inv), keepf("sr_frac", sr_frac);
     keepf("dec_frac", dec_frac), keepf("dec_B_frac", dec_B_frac);
 
-    // int8-MFMA operand tables (transposed matrices, byte-reversed balanced digits)
-    auto mfma_table = [&](int K, int J, auto entry) {
+    // int8-MFMA operand tables.  For a K x J matrix M mod p_j: entry
+    // E[j][v][i] packs, in byte u, digit v of the balanced digits of
+    // (2^(8u) M_ij mod p_j); then per j the 128-bit offset p_j*ceil(2^76/p_j)
+    // that makes the signed digit sum non-negative.  Padded to 16 ceil(J/16)
+    // rows and 8 ceil(K/8) columns with zeros.
+    auto mfma_table = [&](int K, int J, const std::vector<uint64_t> &pj, auto entry) {
         const int KP = 8 * ((K + 7) / 8), JP = 16 * ((J + 15) / 16);
-        std::vector<uint64_t> v((size_t)JP * KP, 0);
-        for (int j = 0; j < J; j++)
-            for (int i = 0; i < K; i++) v[(size_t)j * KP + i] = kb_host_digits_reversed(entry(i, j));
+        std::vector<uint64_t> v((size_t)JP * 8 * KP + 2 * (size_t)JP, 0);
+        for (int j = 0; j < J; j++) {
+            for (int i = 0; i < K; i++) {
+                const uint64_t m = entry(i, j) % pj[j];
+                for (int u = 0; u < 8; u++) {
+                    const uint64_t mu = (uint64_t)(((kb_u128_host)m << (8 * u)) % pj[j]);
+                    const uint64_t d = (mu + KB_DIGIT_BIAS) ^ KB_DIGIT_BIAS;
+                    for (int w = 0; w < 8; w++)
+                        v[((size_t)j * 8 + w) * KP + i] |= ((d >> (8 * w)) & 0xff) << (8 * u);
+                }
+            }
+            const kb_u128_host two76 = (kb_u128_host)1 << 76;
+            const kb_u128_host off = ((two76 + pj[j] - 1) / pj[j]) * pj[j];
+            v[(size_t)JP * 8 * KP + 2 * j] = (uint64_t)off;
+            v[(size_t)JP * 8 * KP + 2 * j + 1] = (uint64_t)(off >> 64);
+        }
         return v;
     };
     const bool mx_ok = rn + 1 <= 32;  // every matrix of the MFMA path has K, J <= 32
     std::vector<uint64_t> mx_qr, mx_rq, mx_sr;
     if (mx_ok) {
-        mx_qr = mfma_table(qn + 1, rn, [&](int i, int j) {
+        mx_qr = mfma_table(qn + 1, rn, R, [&](int i, int j) {
             return i < qn ? q_hat_mod_r[j * qn + i] : (R[j] - alphaQ[rn + j]) % R[j];
         });
-        mx_rq = mfma_table(rn + 1, qn, [&](int i, int j) {
+        mx_rq = mfma_table(rn + 1, qn, Q, [&](int i, int j) {
             return i < rn ? r_hat_mod_q[j * rn + i] : (Q[j] - alphaR[qn + j]) % Q[j];
         });
-        mx_sr = mfma_table(qn, rn, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; });
+        mx_sr = mfma_table(qn, rn, R, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; });
     }
 
     // device upload: one blob for the small tables

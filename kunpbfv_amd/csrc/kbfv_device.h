@@ -92,3 +92,28 @@ __device__ __forceinline__ uint64_t kb_submod(uint64_t a, uint64_t b, uint64_t q
 __device__ __forceinline__ uint64_t kb_f64_to_u64_sat(double x) {
     return x >= 18446744073709551616.0 ? ~0ULL : (uint64_t)x;
 }
+
+// Shoup lazy product from 32-bit halves: r = x*w - floor(x*ws / 2^64) * q in
+// [0, 2q) for any x < 2^64, with nq = 2^64 - q so that both low products
+// accumulate in one 64-bit chain (x*w + h*nq = x*w - h*q mod 2^64).
+__device__ __forceinline__ uint64_t kb_mad32(uint32_t a, uint32_t b, uint64_t c) {
+    return (uint64_t)a * b + c;
+}
+__device__ __forceinline__ uint64_t kb_shoup_lazy_nq(uint64_t x, uint64_t w, uint64_t ws, uint64_t nq) {
+    const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+    const uint32_t s0 = (uint32_t)ws, s1 = (uint32_t)(ws >> 32);
+    const uint64_t t = kb_mad32(x1, s0, __umulhi(x0, s0));
+    const uint64_t u = kb_mad32(x0, s1, (uint32_t)t);
+    const uint64_t h = kb_mad32(x1, s1, (t >> 32) + (u >> 32));  // floor(x*ws / 2^64)
+    const uint32_t h0 = (uint32_t)h, h1 = (uint32_t)(h >> 32);
+    const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+    const uint32_t n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+    const uint64_t r = kb_mad32(h0, n0, kb_mad32(x0, w0, 0));
+    const uint32_t cross = x0 * w1 + x1 * w0 + h0 * n1 + h1 * n0;
+    return r + ((uint64_t)cross << 32);
+}
+// a >= b ? a - b : a, for a, b < 2^62 (the sign of the difference decides)
+__device__ __forceinline__ uint64_t kb_csub_s(uint64_t a, uint64_t b) {
+    const uint64_t t = a - b;
+    return (int64_t)t < 0 ? a : t;
+}

@@ -80,8 +80,8 @@ struct kb_dev_tables {
     const kbfv_tw *p_hatinv;         // [pn]
     const kbfv_tw *Pinv_mod_q;       // [qn]
     const uint64_t *P_mod_q;         // [qn]
-    // int8-MFMA operand tables: byte-reversed balanced digits of the transposed
-    // matrices, [16*ceil(J/16)][8*ceil(K/8)], zero padded (null if too large)
+    // int8-MFMA operand tables: [16*ceil(J/16)][8][8*ceil(K/8)] packed digits of
+    // 2^(8u) M_ij mod p_j, then [16*ceil(J/16)][2] offsets (null if too large)
     const uint64_t *mx_qr;  // base conversion Q -> R, K = qn + 1 (alpha row [-Q]_r)
     const uint64_t *mx_rq;  // base conversion R -> Q, K = rn + 1 (alpha row [-R]_q)
     const uint64_t *mx_sr;  // scale-and-round, K = qn

@@ -17,6 +17,13 @@
 // which is exact in 128-bit wrap-around arithmetic because the true value is a
 // non-negative integer below 2^127.
 //
+// Constant matrices use an expanded form instead: since x = sum_u d_u 2^(8u),
+//     sum_k x_k m_k = sum_(k,u) d_ku (2^(8u) m_k mod p)   (mod p),
+// so K runs over (k, u) with the data digits as they are and the matrix side
+// holds the balanced digits v = 0..7 of the precomputed 2^(8u) m_k mod p: 8
+// MFMA passes (one per v) instead of 15, no zero bytes, and the signed sum
+// (|.| < K 2^70) is made non-negative with a per-row offset, a multiple of p.
+//
 // MFMA v_mfma_i32_16x16x64_i8 lane layout (checked on the device): lane l holds
 // row (A) / column (B) l % 16 and K bytes 16 (l / 16) .. 16 (l / 16) + 15; the
 // result lane l holds rows 4 (l / 16) + i, i = 0..3, of column l % 16.  Here a
@@ -57,7 +64,3 @@ __device__ __forceinline__ void kb_horner48(uint64_t &lo, uint64_t &hi, int64_t 
     lo = nl;
 }
 
-// matrix-side table entry: the byte-reversed digits of m (host side)
-__host__ static inline uint64_t kb_host_digits_reversed(uint64_t m) {
-    return __builtin_bswap64((m + KB_DIGIT_BIAS) ^ KB_DIGIT_BIAS);
-}

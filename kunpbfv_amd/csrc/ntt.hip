@@ -38,18 +38,17 @@ struct Geo {
     }
 };
 
-__device__ __forceinline__ void ct_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t q,
-                                        uint64_t q2) {
-    a = kb_csub(a, q2);                          // [0, 2q)
-    uint64_t t = kb_shoup_lazy(b, w.w, w.ws, q); // [0, 2q)
+// values stay below 4q < 2^62; nq = 2^64 - q
+__device__ __forceinline__ void ct_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t nq, uint64_t q2) {
+    a = kb_csub_s(a, q2);                             // [0, 2q)
+    const uint64_t t = kb_shoup_lazy_nq(b, w.w, w.ws, nq);  // [0, 2q)
     b = a + q2 - t;
     a = a + t;
 }
 
-__device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t q,
-                                        uint64_t q2) {
-    uint64_t u = kb_csub(a + b, q2);
-    uint64_t v = kb_shoup_lazy(a + q2 - b, w.w, w.ws, q);
+__device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t nq, uint64_t q2) {
+    const uint64_t u = kb_csub_s(a + b, q2);
+    const uint64_t v = kb_shoup_lazy_nq(a + q2 - b, w.w, w.ws, nq);
     a = u;
     b = v;
 }
@@ -69,7 +68,7 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
     using G = Geo<LOG_S, COLUMN>;
     constexpr int S = G::S, C = G::C, RR = 1 << LR;
     constexpr int NG = C * (S >> LR);
-    const uint64_t q2 = q << 1;
+    const uint64_t q2 = q << 1, nq = 0 - q;
     const int Bs = S >> lam0;
     const int d = Bs >> LR;
     for (int grp = threadIdx.x; grp < NG; grp += G::T) {
@@ -104,7 +103,7 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
                     const int pos = posc + (kbase + i0 * d) * pstep;
                     const kbfv_tw w = tab[(1 << gst) + (pos >> (log_n - gst))];
 #pragma unroll
-                    for (int i = i0; i < i0 + h; ++i) ct_bfly(x[i], x[i + h], w, q, q2);
+                    for (int i = i0; i < i0 + h; ++i) ct_bfly(x[i], x[i + h], w, nq, q2);
                 }
             }
         } else {
@@ -119,8 +118,8 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
                     const kbfv_tw w = tab[(1 << gst) + (pos >> (log_n - gst))];
 #pragma unroll
                     for (int i = i0; i < i0 + h; ++i) {
-                        gs_bfly(x[i], x[i + h], w, q, q2);
-                        if (NINV && gst == 0) x[i] = kb_shoup_lazy(x[i], ninv.w, ninv.ws, q);
+                        gs_bfly(x[i], x[i + h], w, nq, q2);
+                        if (NINV && gst == 0) x[i] = kb_shoup_lazy_nq(x[i], ninv.w, ninv.ws, nq);
                     }
                 }
             }

@@ -8,6 +8,11 @@
 // B operand (columns = positions p); a result lane then holds 16 consecutive
 // positions of 4 output limbs and its stores are 128-byte runs.
 //
+// The matrix enters in expanded form (kbfv_mfma.h): x_i = sum_u d_iu 2^(8u)
+// gives sum_i x_i M_ij = sum_(i,u) d_iu (2^(8u) M_ij mod p_j)  (mod p_j), so
+// K runs over (i, u) and the products split into 8 digit passes of the
+// expanded entries -- 8 MFMA passes and 8 int32 partial sums per output.
+//
 // Base conversion Q -> R (and R -> Q):  y_i = [x_i (Q/q_i)^-1]_{q_i};
 //   alpha = llround(sum_i y_i / q_i)   (double, input order, explicit fma);
 //   out_j = [sum_i y_i [Q/q_i]_{r_j} + alpha [-Q]_{r_j}]_{r_j}
@@ -22,14 +27,16 @@
 
 namespace {
 
-constexpr int RG_MT = 2;                 // 16-position tiles per wave
-constexpr int RG_POS = 16 * RG_MT;       // positions per wave
+constexpr int RG_MT = 4;                 // 16-position tiles per wave
+constexpr int RG_POS = 16 * RG_MT;       // positions per wave (one per lane in the estimate)
 constexpr int RG_WAVES = 4;              // waves per workgroup
 constexpr int RG_BLOCK = RG_POS * RG_WAVES;
 
 // KC: 8-row chunks of K (K <= 8 KC);  NJ: 16-row tiles of output limbs.
 // SR = false: base conversion (Shoup-scaled inputs, alpha row at K index in_n);
 // SR = true: scale-and-round (raw Q inputs, R-limb diagonal term and nu added after).
+// mat: [16 NJ][8][8 KC] packed digits of 2^(8u) M_ij mod p_j, then [16 NJ][2]
+// offsets (the expanded-matrix form of kbfv_mfma.h).
 template <int KC, int NJ, bool SR>
 __global__ void __launch_bounds__(64 * RG_WAVES, 2)
     k_rns_gemm(const uint64_t *__restrict__ x, long in_stride, uint64_t *__restrict__ y, long out_stride,
@@ -37,7 +44,7 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
                const uint64_t *__restrict__ mus_out, const kbfv_tw *__restrict__ hatinv,
                const uint64_t *__restrict__ mods_in, const double *__restrict__ est,
                const uint64_t *__restrict__ sr_rows, int in_n, int out_n, int n) {
-    constexpr int KP = 8 * KC;
+    constexpr int KP = 8 * KC, JP = 16 * NJ;
     __shared__ uint64_t ys[RG_WAVES][KP][RG_POS];
     __shared__ uint64_t alpha_s[RG_WAVES][RG_POS];
     const int lane = threadIdx.x & 63;
@@ -74,10 +81,9 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
     __syncthreads();
     // the reference's double estimate, one position per lane, in input order
     {
-        const int pp = lane & (RG_POS - 1);
         double acc = SR ? 0.5 : 0.0;
-        for (int i = 0; i < in_n; ++i) acc = fma((double)ys[wave][i][pp], est[i], acc);
-        if (lane < RG_POS) alpha_s[wave][pp] = SR ? kb_f64_to_u64_sat(acc) : (uint64_t)(int)llround(acc);
+        for (int i = 0; i < in_n; ++i) acc = fma((double)ys[wave][i][lane], est[i], acc);
+        alpha_s[wave][lane] = SR ? kb_f64_to_u64_sat(acc) : (uint64_t)(int)llround(acc);
     }
     __syncthreads();
 
@@ -95,31 +101,26 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
             bop[mt][c] = kb_pack2(kb_digits(d0), kb_digits(d1));
         }
     }
-    // one 16-limb output tile at a time: its matrix rows (byte-reversed digits),
-    // 15 weight passes folded by Horner, then the reduction and the stores
+    const uint64_t *offs = mat + (long)JP * 8 * KP;
+    // one 16-limb output tile at a time: 8 digit passes of the expanded matrix,
+    // Horner in 2^8 into two int64 halves, then offset, reduction and stores
 #pragma unroll 1
     for (int nt = 0; nt < NJ; ++nt) {
-        uint64_t rm[KC][2];
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-            const uint64_t *mr = mat + (long)(16 * nt + p16) * KP + 8 * c + 2 * g;
-            rm[c][0] = mr[0];
-            rm[c][1] = mr[1];
-        }
-        int64_t S[RG_MT][4];
-        uint64_t Hl[RG_MT][4], Hh[RG_MT][4];
+        const kb_v4i *mrow = (const kb_v4i *)(mat + (long)(16 * nt + p16) * 8 * KP + 8 * 0 + 2 * g);
+        int64_t Shi[RG_MT][4], Slo[RG_MT][4];
 #pragma unroll
         for (int mt = 0; mt < RG_MT; ++mt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) S[mt][i] = 0, Hl[mt][i] = 0, Hh[mt][i] = 0;
+            for (int i = 0; i < 4; ++i) Shi[mt][i] = 0, Slo[mt][i] = 0;
 #pragma unroll
-        for (int w = 14; w >= 0; --w) {
+        for (int v = 7; v >= 0; --v) {
             kb_v4i acc[RG_MT];
 #pragma unroll
             for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_v4i{0, 0, 0, 0};
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
-                const kb_v4i a = kb_pack2(kb_weight_op(rm[c][0], w), kb_weight_op(rm[c][1], w));
+                // E[j][v][8c + 2g .. +1]: 16 bytes, row stride 8 KP words, v stride KP words
+                const kb_v4i a = mrow[(v * KP + 8 * c) / 2];
 #pragma unroll
                 for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_mfma_i8(a, bop[mt][c], acc[mt]);
             }
@@ -127,28 +128,34 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
             for (int mt = 0; mt < RG_MT; ++mt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    S[mt][i] = (int64_t)((uint64_t)S[mt][i] << 8) + (int64_t)acc[mt][i];
-                    if (w == 12 || w == 6 || w == 0) {
-                        kb_horner48(Hl[mt][i], Hh[mt][i], S[mt][i]);
-                        S[mt][i] = 0;
-                    }
+                    if (v >= 4)
+                        Shi[mt][i] = (int64_t)((uint64_t)Shi[mt][i] << 8) + (int64_t)acc[mt][i];
+                    else
+                        Slo[mt][i] = (int64_t)((uint64_t)Slo[mt][i] << 8) + (int64_t)acc[mt][i];
                 }
-                // keep each weight pass together (no hoisting of later passes' operands)
-                __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int j = 16 * nt + 4 * g + i;
             if (j >= out_n) continue;
             const uint64_t pj = mods_out[j], mlo = mus_out[2 * j], mhi = mus_out[2 * j + 1];
+            const uint64_t olo = offs[2 * j], ohi = offs[2 * j + 1];
 #pragma unroll
             for (int mt = 0; mt < RG_MT; ++mt) {
-                kbfv_u128 v{Hl[mt][i], Hh[mt][i]};
+                // Shi 2^32 + Slo + offset (signed halves, non-negative total < 2^78)
+                uint64_t lo = (uint64_t)Shi[mt][i] << 32;
+                uint64_t hi = (uint64_t)(Shi[mt][i] >> 32);
+                const uint64_t sl = (uint64_t)Slo[mt][i];
+                uint64_t t = lo + sl;
+                hi += (uint64_t)(t < lo) + (Slo[mt][i] < 0 ? ~0ULL : 0ULL);
+                lo = t + olo;
+                hi += (uint64_t)(lo < t) + ohi;
+                kbfv_u128 vv{lo, hi};
                 if (SR) {
-                    kb_mac128(v, xp[(long)(in_n + j) * n + 16 * mt + p16], sr_rows[(long)j * (in_n + 1) + in_n]);
-                    kb_add128(v, alpha_s[wave][16 * mt + p16]);
+                    kb_mac128(vv, xp[(long)(in_n + j) * n + 16 * mt + p16], sr_rows[(long)j * (in_n + 1) + in_n]);
+                    kb_add128(vv, alpha_s[wave][16 * mt + p16]);
                 }
-                yp[(long)j * n + 16 * mt + p16] = kb_barrett128(v, pj, mlo, mhi);
+                yp[(long)j * n + 16 * mt + p16] = kb_barrett128(vv, pj, mlo, mhi);
             }
         }
     }

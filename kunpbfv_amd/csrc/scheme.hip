@@ -35,7 +35,9 @@ enum : uint64_t {
     TAG_ENC_E0 = 7,
     TAG_ENC_E1 = 8,
     TAG_RGSW_A = 9,
-    TAG_RGSW_E = 10
+    TAG_RGSW_E = 10,
+    TAG_GAL_A = 11,
+    TAG_GAL_E = 12
 };
 
 // bump allocator over the caller's workspace (256-byte aligned slices); with a
@@ -97,10 +99,12 @@ __global__ void k_dec_combine(kb_dev_tables t, const uint64_t *__restrict__ ct,
     x[i] = kb_addmod(ct[bt * 2 * L + L + r], kb_mulmod(ct[bt * 2 * L + r], s_q[r], q, mlo, mhi), q);
 }
 
-// relinearisation key body for digit d: b = e + P*s^2*[limbs of digit d] - a*s  (over PQ)
-__global__ void k_rlk_combine(kb_dev_tables t, const uint64_t *__restrict__ e,
-                              const uint64_t *__restrict__ a, const uint64_t *__restrict__ s_pq,
-                              const uint64_t *__restrict__ s2_q, uint64_t *__restrict__ b, int pn, int qn,
+// key-switching key body for digit d (over PQ):
+//   b = e + P*src*[limbs of digit d] - a*keypoly
+// relinearisation: src = s^2, keypoly = s; Galois g: src = s, keypoly = sigma_g^-1(s)
+__global__ void k_ksk_combine(kb_dev_tables t, const uint64_t *__restrict__ e,
+                              const uint64_t *__restrict__ a, const uint64_t *__restrict__ key_pq,
+                              const uint64_t *__restrict__ src_q, uint64_t *__restrict__ b, int pn, int qn,
                               int n, long total) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -112,10 +116,32 @@ __global__ void k_rlk_combine(kb_dev_tables t, const uint64_t *__restrict__ e,
     uint64_t v = e[i];
     const int ql = l - pn;
     if (ql >= 0 && ql >= (int)d * pn && ql < (int)d * pn + pn) {
-        const uint64_t ps2 = kb_mulmod(s2_q[(long)ql * n + (r % n)], t.P_mod_q[ql], q, mlo, mhi);
-        v = kb_addmod(v, ps2, q);
+        const uint64_t ps = kb_mulmod(src_q[(long)ql * n + (r % n)], t.P_mod_q[ql], q, mlo, mhi);
+        v = kb_addmod(v, ps, q);
     }
-    b[i] = kb_submod(v, kb_mulmod(a[i], s_pq[r], q, mlo, mhi), q);
+    b[i] = kb_submod(v, kb_mulmod(a[i], key_pq[r], q, mlo, mhi), q);
+}
+
+// coefficient-form automorphism X -> X^g (gather form, g^-1 = ginv mod 2N):
+// out_j = in_i for i = j ginv mod 2N < N, else -in_(i - N)
+__global__ void k_automorphism(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                               uint64_t ginv, int mod0, int limbs, int log_n, long total) {
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const long n = 1L << log_n;
+    const long row = idx >> log_n;
+    const uint64_t j = (uint64_t)(idx & (n - 1));
+    const uint64_t i = (j * ginv) & (2 * (uint64_t)n - 1);
+    const uint64_t q = t.mod[mod0 + (int)(row % limbs)];
+    const uint64_t *src = in + row * n;
+    uint64_t v;
+    if (i < (uint64_t)n) {
+        v = src[i];
+    } else {
+        const uint64_t x = src[i - n];
+        v = x ? q - x : 0;
+    }
+    out[idx] = v;
 }
 
 // signed lift of a plaintext polynomial m [N] (values above t/2 negative) into Q
@@ -360,7 +386,7 @@ extern "C" int kbfv_keygen_rlk(const kbfv_context *c, const uint8_t seed[32], co
     KB_TRY(kb_ntt_inplace(c, +1, e, c->oP, pq, D, s));
     KB_TRY(kb_sample(c, 0, seed, TAG(TAG_RLK_A), 0, c->oP, pq, a, 0, D, s));
     const long total = (long)D * pq * N;
-    hipLaunchKernelGGL(k_rlk_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e, a, sk, s2, b, pn, qn,
+    hipLaunchKernelGGL(k_ksk_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e, a, sk, s2, b, pn, qn,
                        c->n, total);
     KB_LAUNCH_CHECK();
     KB_TRY(kb_ntt_inplace(c, -1, a, c->oP, pq, D, s));
@@ -453,6 +479,131 @@ extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const
     for (int b0 = 0; b0 < batch; b0 += cb) {
         const int n = std::min(cb, batch - b0);
         KB_TRY(mult_chunk(c, rlk, ct1 + b0 * CTS, ct2 + b0 * CTS, out + b0 * CTS, n, w, S(st)));
+    }
+    return KBFV_OK;
+}
+
+// ---------------------------------------------------------------- Galois automorphisms
+// (BFVScheme::addLeftRotKey_23 / addConjKey_23 / leftRotateAndEqual_23 /
+// conjugate_23, include/KeySwitch_23.cuh:70-166, 466-584)
+static uint64_t inv_mod_2n(uint64_t g, uint64_t m2) {
+    uint64_t x = g;  // g * g = 1 mod 8; each Newton step doubles the correct bits
+    for (int i = 0; i < 6; ++i) x *= 2 - g * x;
+    return x & (m2 - 1);
+}
+
+static int launch_automorphism(const kbfv_context *c, uint64_t g, const uint64_t *in, uint64_t *out, int mod0,
+                               int limbs, int polys, hipStream_t s) {
+    const uint64_t m2 = 2 * (uint64_t)c->n;
+    KB_REQUIRE((g & 1) && g < m2, "Galois element must be odd and below 2N");
+    KB_REQUIRE(in != out, "automorphism cannot run in place");
+    KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod && polys > 0, "bad limb range");
+    const long total = (long)polys * limbs * c->n;
+    hipLaunchKernelGGL(k_automorphism, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, in, out,
+                       inv_mod_2n(g, m2), mod0, limbs, c->log_n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+extern "C" uint64_t kbfv_galois_elt(const kbfv_context *c, int64_t steps) {
+    if (!c) return 0;
+    const uint64_t m2 = 2 * (uint64_t)c->n;
+    if (steps < 0) return m2 - 1;  // conjugation X -> X^-1
+    uint64_t g = 1;
+    for (int64_t i = 0; i < steps % (c->n / 2); ++i) g = g * 5 % m2;  // 5 has order N/2 mod 2N
+    return g;
+}
+
+extern "C" int kbfv_automorphism(const kbfv_context *c, uint64_t g, const uint64_t *in, uint64_t *out,
+                                 int mod0, int limbs, int polys, void *st) {
+    KB_REQUIRE(c && in && out, "null argument");
+    return launch_automorphism(c, g, in, out, mod0, limbs, polys, S(st));
+}
+
+static size_t gal_layout(const kbfv_context *c, void *base, uint64_t **kc, uint64_t **kp, uint64_t **e,
+                         uint64_t **a, uint64_t **b) {
+    const size_t N = c->n, pq = c->pn + c->qn, D = c->dnum;
+    Bump bp(base);
+    uint64_t *p0 = bp.take(pq * N), *p1 = bp.take(pq * N), *p2 = bp.take(D * pq * N), *p3 = bp.take(D * pq * N),
+             *p4 = bp.take(D * pq * N);
+    if (kc) *kc = p0, *kp = p1, *e = p2, *a = p3, *b = p4;
+    return bp.used;
+}
+
+extern "C" size_t kbfv_keygen_galois_workspace_bytes(const kbfv_context *c) {
+    return c ? gal_layout(c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) : 0;
+}
+
+extern "C" int kbfv_keygen_galois(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk, uint64_t g,
+                                  uint64_t *key, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && seed && sk && key && ws, "null argument");
+    KB_REQUIRE(ws_bytes >= kbfv_keygen_galois_workspace_bytes(c), "workspace too small");
+    const uint64_t m2 = 2 * (uint64_t)c->n;
+    KB_REQUIRE((g & 1) && g < m2, "Galois element must be odd and below 2N");
+    uint64_t *kc, *kp, *e, *a, *b;
+    gal_layout(c, ws, &kc, &kp, &e, &a, &b);
+    const long N = c->n;
+    const int pn = c->pn, qn = c->qn, pq = pn + qn, D = c->dnum, B = c->ri_blocks, tn = c->tn;
+    hipStream_t s = S(st);
+    // key polynomial sigma_g^-1(s) over P u Q, NTT form
+    KB_TRY(kb_ntt(c, -1, sk, kc, c->oP, pq, 1, 1, 0, (long)pq * N, N, 0, (long)pq * N, N, s));
+    KB_TRY(launch_automorphism(c, inv_mod_2n(g, m2), kc, kp, c->oP, pq, 1, s));
+    KB_TRY(kb_ntt_inplace(c, +1, kp, c->oP, pq, 1, s));
+    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_GAL_E) | g, 0, c->oP, pq, e, 0, D, s));
+    KB_TRY(kb_ntt_inplace(c, +1, e, c->oP, pq, D, s));
+    KB_TRY(kb_sample(c, 0, seed, TAG(TAG_GAL_A) | g, 0, c->oP, pq, a, 0, D, s));
+    const long total = (long)D * pq * N;
+    hipLaunchKernelGGL(k_ksk_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e, a, kp,
+                       sk + (long)pn * N, b, pn, qn, c->n, total);
+    KB_LAUNCH_CHECK();
+    KB_TRY(kb_ntt_inplace(c, -1, a, c->oP, pq, D, s));
+    KB_TRY(kb_ntt_inplace(c, -1, b, c->oP, pq, D, s));
+    const long half = (long)D * B * tn * N;
+    KB_TRY(kb_modup_pq_to_t(c, a, key, D, s));
+    KB_TRY(kb_modup_pq_to_t(c, b, key + half, D, s));
+    return kb_ntt_inplace(c, +1, key, c->oT, tn, 2 * D * B, s);
+}
+
+struct RotWs {
+    uint64_t *X, *CT, *IP, *PQ, *MD;
+};
+static size_t rot_layout(const kbfv_context *c, int cb, void *base, RotWs *w) {
+    const size_t N = c->n, qn = c->qn, pq = c->pn + c->qn;
+    Bump bp(base);
+    uint64_t *x = bp.take(2 * qn * N * cb), *ct = bp.take((size_t)c->dnum * c->tn * N * cb),
+             *ip = bp.take(2 * (size_t)c->ri_blocks * c->tn * N * cb), *pqb = bp.take(2 * pq * N * cb),
+             *md = bp.take(2 * qn * N * cb);
+    if (w) *w = RotWs{x, ct, ip, pqb, md};
+    return bp.used;
+}
+
+extern "C" size_t kbfv_rotate_workspace_bytes(const kbfv_context *c, int batch) {
+    return (c && batch > 0) ? rot_layout(c, batch, nullptr, nullptr) : 0;
+}
+
+// ct -> sigma_g(KS(a) + (0, b)): key switching from sigma_g^-1(s) to s on the
+// coefficient-form a, then the automorphism, then back to the NTT domain
+extern "C" int kbfv_rotate(const kbfv_context *c, const uint64_t *gkey, uint64_t g, const uint64_t *ct,
+                           uint64_t *out, int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && gkey && ct && out && ws && batch > 0, "null argument");
+    int cb = batch;
+    while (cb > 1 && rot_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
+    KB_REQUIRE(rot_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
+    RotWs w;
+    rot_layout(c, cb, ws, &w);
+    const long N = c->n, qn = c->qn, CTS = 2 * qn * N;
+    hipStream_t s = S(st);
+    for (int b0 = 0; b0 < batch; b0 += cb) {
+        const int nb = std::min(cb, batch - b0);
+        KB_TRY(kb_ntt(c, -1, ct + b0 * CTS, w.X, c->oQ, qn, 2, nb, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_digits_to_t_ntt(c, w.X, CTS, w.CT, nb, nullptr, s));
+        KB_TRY(kb_inner_product(c, w.CT, gkey, w.IP, nb, s));
+        KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * nb, 1, 0, (long)c->tn * N, N, 0,
+                      (long)c->tn * N, N, s));
+        KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, nb, s));
+        KB_TRY(kb_moddown(c, w.PQ, w.MD, w.X + qn * N, CTS, nb, s));
+        KB_TRY(launch_automorphism(c, g, w.MD, w.X, c->oQ, qn, 2 * nb, s));
+        KB_TRY(kb_ntt(c, +1, w.X, out + b0 * CTS, c->oQ, qn, 2, nb, CTS, qn * N, N, CTS, qn * N, N, s));
     }
     return KBFV_OK;
 }
