@@ -214,6 +214,14 @@ void kbfv_graph_destroy(kbfv_graph *graph);
  * In place on [batch][N]. */
 int kbfv_encode(const kbfv_context *ctx, uint64_t *vals, int batch, void *stream);
 int kbfv_decode(const kbfv_context *ctx, uint64_t *vals, int batch, void *stream);
+/* The same in SEAL's slot order (BFVContext::preComputeIndex with
+ * encode_BFV_kernel / decode_BFV_kernel, include/BFVcontext.cuh:102-129,
+ * include/Encoder.cuh:6-21, which the reference computes but never calls):
+ * two rows of N/2 slots, so the Galois element 5^r rotates each row left by r
+ * and 2N - 1 swaps the rows.  In place on vals [batch][N]; tmp: [batch][N]
+ * scratch.  Needs batching (t = 1 mod 2N). */
+int kbfv_encode_seal(const kbfv_context *ctx, uint64_t *vals, uint64_t *tmp, int batch, void *stream);
+int kbfv_decode_seal(const kbfv_context *ctx, uint64_t *vals, uint64_t *tmp, int batch, void *stream);
 
 /* ---------------------------------------------------------------- Galois automorphisms */
 

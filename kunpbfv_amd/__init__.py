@@ -61,6 +61,8 @@ SIGNATURES = {
     "kbfv_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_encode_seal": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_decode_seal": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_galois_elt": (_u64, [_vp, _i64]),
     "kbfv_automorphism": (C.c_int, [_vp, _u64, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_keygen_galois_workspace_bytes": (_sz, [_vp]),
@@ -353,6 +355,17 @@ class Context:
         check(lib.kbfv_mult_relin(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
                                   stream_ptr(stream)))
         return out
+
+    def encode_seal(self, vals, stream=None):
+        """slots in SEAL's order -> plaintext coefficients, in place"""
+        tmp = self.empty(*vals.shape)
+        check(lib.kbfv_encode_seal(self.h, ptr(vals), ptr(tmp), vals.shape[0], stream_ptr(stream)))
+        return vals
+
+    def decode_seal(self, vals, stream=None):
+        tmp = self.empty(*vals.shape)
+        check(lib.kbfv_decode_seal(self.h, ptr(vals), ptr(tmp), vals.shape[0], stream_ptr(stream)))
+        return vals
 
     # ---- Galois automorphisms / rotation
     def galois_elt(self, steps):

@@ -543,6 +543,19 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
         mx_sr = mfma_table(qn, rn, R, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; });
     }
 
+    // SEAL slot order (include/BFVcontext.cuh:102-129): row 0 slot i at the NTT
+    // index of psi^(5^i), row 1 at that of psi^(-5^i)
+    std::vector<uint64_t> seal_map;
+    if (c->batching) {
+        seal_map.resize(N);
+        uint64_t pos = 1, m2 = 2 * (uint64_t)N;
+        for (int i = 0; i < N / 2; i++) {
+            seal_map[i] = bitrev((pos - 1) >> 1, log_n);
+            seal_map[N / 2 + i] = bitrev((m2 - pos - 1) >> 1, log_n);
+            pos = pos * 5 % m2;
+        }
+    }
+
     // device upload: one blob for the small tables
     Blob blob;
     std::vector<uint64_t> tvec_mod_q(Q), tvec_plain(qn, plain);
@@ -563,6 +576,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     size_t o_half = blob.add(halfT), o_phq = blob.add(p_hat_mod_q), o_phi = blob.add(tws(p_hatinv, P));
     size_t o_pinv = blob.add(tws(Pinv, Q)), o_pmq = blob.add(P_mod_q);
     size_t o_mxqr = blob.add(mx_qr), o_mxrq = blob.add(mx_rq), o_mxsr = blob.add(mx_sr);
+    size_t o_seal = blob.add(seal_map);
 
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipMalloc(&c->d_blob, blob.bytes.size());
@@ -596,6 +610,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     d.t_mod_pq = (const uint64_t *)at(o_tmp), d.halfT_mod = (const uint64_t *)at(o_half);
     d.p_hat_mod_q = (const uint64_t *)at(o_phq), d.p_hatinv = (const kbfv_tw *)at(o_phi);
     d.Pinv_mod_q = (const kbfv_tw *)at(o_pinv), d.P_mod_q = (const uint64_t *)at(o_pmq);
+    if (c->batching) d.seal_map = (const uint64_t *)at(o_seal);
     if (mx_ok) {
         d.mx_qr = (const uint64_t *)at(o_mxqr), d.mx_rq = (const uint64_t *)at(o_mxrq);
         d.mx_sr = (const uint64_t *)at(o_mxsr);

@@ -85,6 +85,8 @@ struct kb_dev_tables {
     const uint64_t *mx_qr;  // base conversion Q -> R, K = qn + 1 (alpha row [-Q]_r)
     const uint64_t *mx_rq;  // base conversion R -> Q, K = rn + 1 (alpha row [-R]_q)
     const uint64_t *mx_sr;  // scale-and-round, K = qn
+    // SEAL slot order: NTT index of slot i (batching only, else null)
+    const uint64_t *seal_map;  // [N]
 };
 
 struct kbfv_context {

@@ -26,7 +26,8 @@ constexpr int IM_THREADS = 64 * IM_POS;      // 512
 constexpr int IM_CT = 32;                    // ciphertexts per tile (2 MFMA column tiles)
 constexpr int IM_ROWS = 32;                  // key rows (h, b) held: 2 x ri_blocks <= 32
 constexpr int IM_KP = 32;                    // K (digits) held: dnum <= 32
-constexpr int IM_PS = IM_ROWS * IM_KP + 2;   // LDS stride between coefficients, [pos][row][d]
+constexpr int IM_RS = IM_KP + 2;             // LDS row stride: 16 lanes' 16-byte reads hit distinct banks
+constexpr int IM_PS = IM_ROWS * IM_RS + 2;   // LDS stride between coefficients, [pos][row][d]
 constexpr int IM_CS = IM_ROWS * IM_POS + 1;  // LDS stride between ciphertexts, [ct][row][pos]
 constexpr int IM_LDS = (IM_POS * IM_PS > IM_CT * IM_CS) ? IM_POS * IM_PS : IM_CT * IM_CS;
 // per-thread share of a digit tile (IM_CT x IM_KP x IM_POS values)
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(IM_THREADS, 2)
         const int hd = rest / B;  // h*D + d
         const int d = hd % D, h = hd / D;
         const uint64_t v = key[((long)hd * B + b) * tn * N + (long)kk * N + n0 + pos];
-        lds[pos * IM_PS + (h * B + b) * IM_KP + d] = v;
+        lds[pos * IM_PS + (h * B + b) * IM_RS + d] = v;
     }
     __syncthreads();
     uint64_t rk[NJ][KC][2];
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(IM_THREADS, 2)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int r = 16 * nt + p16, d = 8 * c + 2 * g + e;
-                const uint64_t v = (r < R2 && d < D) ? lds[wave * IM_PS + r * IM_KP + d] : 0;
+                const uint64_t v = (r < R2 && d < D) ? lds[wave * IM_PS + r * IM_RS + d] : 0;
                 rk[nt][c][e] = __builtin_bswap64(kb_digits(v));
             }
 
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(IM_THREADS, 2)
             const int pos = e % IM_POS;
             const int d = (e / IM_POS) % IM_KP;
             const int ctl = e / (IM_POS * IM_KP);
-            lds[pos * IM_PS + ctl * IM_KP + d] = pre[r];
+            lds[pos * IM_PS + ctl * IM_RS + d] = pre[r];
         }
         __syncthreads();
         kb_v4i bop[2][KC];
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(IM_THREADS, 2)
         for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
-                const uint64_t *src = lds + wave * IM_PS + (16 * mt + p16) * IM_KP + 8 * c + 2 * g;
+                const uint64_t *src = lds + wave * IM_PS + (16 * mt + p16) * IM_RS + 8 * c + 2 * g;
                 bop[mt][c] = kb_pack2(kb_digits(src[0]), kb_digits(src[1]));
             }
         __syncthreads();  // the tile's digits are in registers; LDS now takes outputs

@@ -122,6 +122,21 @@ __global__ void k_ksk_combine(kb_dev_tables t, const uint64_t *__restrict__ e,
     b[i] = kb_submod(v, kb_mulmod(a[i], key_pq[r], q, mlo, mhi), q);
 }
 
+// slot permutation over [batch][N]: SCATTER out[map[i]] = in[i], else out[i] = in[map[i]]
+template <bool SCATTER>
+__global__ void k_slot_permute(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                               const uint64_t *__restrict__ map, int log_n, long total) {
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const long n = 1L << log_n;
+    const long base = idx & ~(n - 1);
+    const long i = idx & (n - 1);
+    if (SCATTER)
+        out[base + (long)map[i]] = in[idx];
+    else
+        out[idx] = in[base + (long)map[i]];
+}
+
 // coefficient-form automorphism X -> X^g (gather form, g^-1 = ginv mod 2N):
 // out_j = in_i for i = j ginv mod 2N < N, else -in_(i - N)
 __global__ void k_automorphism(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
@@ -613,6 +628,29 @@ extern "C" int kbfv_encode(const kbfv_context *c, uint64_t *vals, int batch, voi
     KB_REQUIRE(c && vals && batch > 0, "null argument");
     if (!c->batching) return KBFV_OK;  // coefficient encoding
     return kb_ntt_inplace(c, -1, vals, c->oPlain, 1, batch, S(st));
+}
+// SEAL slot order (BFVContext::preComputeIndex, include/BFVcontext.cuh:102-129;
+// encode_BFV_kernel / decode_BFV_kernel, include/Encoder.cuh:6-21)
+extern "C" int kbfv_encode_seal(const kbfv_context *c, uint64_t *vals, uint64_t *tmp, int batch, void *st) {
+    KB_REQUIRE(c && vals && tmp && batch > 0 && vals != tmp, "null or aliased argument");
+    KB_REQUIRE(c->batching && c->t.seal_map, "the SEAL slot order needs t = 1 mod 2N");
+    const long total = (long)batch * c->n;
+    hipLaunchKernelGGL(k_slot_permute<true>, dim3(blocks_for(total, BS)), dim3(BS), 0, S(st), vals, tmp,
+                       c->t.seal_map, c->log_n, total);
+    KB_LAUNCH_CHECK();
+    const long N = c->n;
+    return kb_ntt(c, -1, tmp, vals, c->oPlain, 1, 1, batch, N, 0, 0, N, 0, 0, S(st));
+}
+extern "C" int kbfv_decode_seal(const kbfv_context *c, uint64_t *vals, uint64_t *tmp, int batch, void *st) {
+    KB_REQUIRE(c && vals && tmp && batch > 0 && vals != tmp, "null or aliased argument");
+    KB_REQUIRE(c->batching && c->t.seal_map, "the SEAL slot order needs t = 1 mod 2N");
+    const long total = (long)batch * c->n, N = c->n;
+    KB_TRY(kb_reduce_plain(c, vals, total, S(st)));
+    KB_TRY(kb_ntt(c, +1, vals, tmp, c->oPlain, 1, 1, batch, N, 0, 0, N, 0, 0, S(st)));
+    hipLaunchKernelGGL(k_slot_permute<false>, dim3(blocks_for(total, BS)), dim3(BS), 0, S(st), tmp, vals,
+                       c->t.seal_map, c->log_n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
 }
 extern "C" int kbfv_decode(const kbfv_context *c, uint64_t *vals, int batch, void *st) {
     KB_REQUIRE(c && vals && batch > 0, "null argument");

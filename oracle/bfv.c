@@ -15,6 +15,7 @@
  *   compute_t_QInv_c0c1c2 (HPS tensoring)     include/KeySwitch_23.cuh:703-926
  *   multAndEqual_23 relinearisation steps     include/KeySwitch_23.cuh:270-305
  *   BFVContext::encode / decode               include/Encoder.cuh:23-113
+ *   BFVContext::preComputeIndex (SEAL order)  include/BFVcontext.cuh:102-129
  * Deviations from the reference: the secret-key shortcut
  * that overwrites the product (include/KeySwitch_23.cuh:308, :823-832) is not
  * reproduced -- the result here is the relinearised tensor product; the NTT of
@@ -312,6 +313,32 @@ void or_encode(const or_ctx *c, const uint64_t *slots, uint64_t *m) {
 void or_decode(const or_ctx *c, const uint64_t *m, uint64_t *slots) {
     for (size_t k = 0; k < c->n; k++) slots[k] = m[k] % c->plain;
     if (c->batching) or_ntt_fwd(slots, c->log_n, c->plain, or_tab(c, c->oPlain, 0));
+}
+
+void or_seal_index_map(const or_ctx *c, uint64_t *map) {
+    size_t n = c->n, row = n / 2;
+    uint64_t m2 = 2 * (uint64_t)n, pos = 1;
+    for (size_t i = 0; i < row; i++) {
+        map[i] = or_bitrev((pos - 1) >> 1, c->log_n);
+        map[row + i] = or_bitrev((m2 - pos - 1) >> 1, c->log_n);
+        pos = pos * 5 % m2;
+    }
+}
+void or_encode_seal(const or_ctx *c, const uint64_t *slots, uint64_t *m) {
+    uint64_t *map = alloc0(c->n), *v = alloc0(c->n);
+    or_seal_index_map(c, map);
+    for (size_t i = 0; i < c->n; i++) v[map[i]] = slots[i];
+    or_encode(c, v, m);
+    free(map);
+    free(v);
+}
+void or_decode_seal(const or_ctx *c, const uint64_t *m, uint64_t *slots) {
+    uint64_t *map = alloc0(c->n), *v = alloc0(c->n);
+    or_seal_index_map(c, map);
+    or_decode(c, m, v);
+    for (size_t i = 0; i < c->n; i++) slots[i] = v[map[i]];
+    free(map);
+    free(v);
 }
 
 /* ---------------- RGSW x RLWE ---------------- */

@@ -135,6 +135,12 @@ void or_add(const or_ctx *c, const uint64_t *a, const uint64_t *b, uint64_t *out
             int mod0, int poly_num);
 void or_encode(const or_ctx *c, const uint64_t *slots, uint64_t *m);
 void or_decode(const or_ctx *c, const uint64_t *m, uint64_t *slots);
+/* SEAL slot order (BFVContext::preComputeIndex, include/BFVcontext.cuh:102-129):
+ * slot i of row 0 sits at NTT index brv((5^i mod 2N - 1) / 2), slot i of row 1
+ * at brv((2N - 5^i mod 2N - 1) / 2); requires batching (t = 1 mod 2N). */
+void or_seal_index_map(const or_ctx *c, uint64_t *map);
+void or_encode_seal(const or_ctx *c, const uint64_t *slots, uint64_t *m);
+void or_decode_seal(const or_ctx *c, const uint64_t *m, uint64_t *slots);
 
 /* Galois automorphisms X -> X^g (g odd, mod 2N) and slot rotation.
  * or_galois_elt: 5^steps mod 2N (the reference's rotGroups, include/BFVcontext.cuh:788-796);

@@ -79,6 +79,9 @@ def _load():
         "or_rgsw_encrypt": (None, [C.c_void_p, u8p, u64p, u64p, C.c_int, u64p]),
         "or_external_product": (None, [C.c_void_p, u64p, u64p, C.c_int, u64p]),
         "or_galois_elt": (C.c_uint64, [C.c_void_p, C.c_long]),
+        "or_seal_index_map": (None, [C.c_void_p, u64p]),
+        "or_encode_seal": (None, [C.c_void_p, u64p, u64p]),
+        "or_decode_seal": (None, [C.c_void_p, u64p, u64p]),
         "or_automorphism": (None, [C.c_void_p, C.c_uint64, u64p, u64p, C.c_int, C.c_int, C.c_int]),
         "or_keygen_galois": (None, [C.c_void_p, u8p, u64p, C.c_uint64, u64p]),
         "or_rotate": (None, [C.c_void_p, u64p, C.c_uint64, u64p, u64p]),
@@ -271,6 +274,17 @@ class Ctx:
         out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
         lib.or_external_product(self.h, P(rg), P(c), base_bits, P(out))
         return out
+
+    def seal_index_map(self):
+        out = np.zeros(self.n, dtype=np.uint64)
+        lib.or_seal_index_map(self.h, P(out))
+        return out
+
+    def encode_seal(self, slots):
+        return self._call(lib.or_encode_seal, [slots], (self.n,))
+
+    def decode_seal(self, m):
+        return self._call(lib.or_decode_seal, [m], (self.n,))
 
     # ---- Galois automorphisms / rotation ----
     def galois_elt(self, steps):

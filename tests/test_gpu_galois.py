@@ -80,3 +80,25 @@ def test_rotate_matches_oracle_and_decrypts(setup, batch):
     # in place (out aliases ct) gives the same bits
     ctx.rotate(key, g, ct, out=ct)
     assert np.array_equal(kb.to_host(ct), roth)
+
+
+def test_seal_order_rotation_is_cyclic(setup):
+    kb, ctx, oc, sk, pk = setup
+    if not oc.batching:
+        pytest.skip("SEAL slot order needs batching")
+    rng = np.random.default_rng(21)
+    batch, row = 2, ctx.n // 2
+    v = rng.integers(0, ctx.plain, size=(batch, ctx.n), dtype=np.uint64)
+    vd = kb.to_device(v)
+    ctx.encode_seal(vd)
+    enc = kb.to_host(vd)
+    for b in range(batch):
+        assert np.array_equal(enc[b], oc.encode_seal(v[b]))
+    g = ctx.galois_elt(1)
+    key = ctx.keygen_galois(bytes([12]) * 32, sk, g)
+    ct = ctx.encrypt(bytes([13]) * 32, pk, vd)
+    dec = ctx.decrypt(sk, ctx.rotate(key, g, ct))
+    ctx.decode_seal(dec)
+    got = kb.to_host(dec)
+    want = np.concatenate([np.roll(v[:, :row], -1, axis=1), np.roll(v[:, row:], -1, axis=1)], axis=1)
+    assert np.array_equal(got, want)

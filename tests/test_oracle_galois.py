@@ -139,3 +139,25 @@ def test_rotation_by_identity_is_keyswitch_only(keys):
     m = np.arange(oc.n, dtype=np.uint64) % oc.plain
     ct = oc.encrypt(bytes([8]) * 32, pk, oc.encode(m))
     assert np.array_equal(oc.decode(oc.decrypt(sk, oc.rotate(gk, 1, ct))), m)
+
+
+def test_seal_order_makes_rotation_cyclic():
+    """In SEAL's slot order the map is a permutation, encode/decode invert each
+    other, and sigma_(5^r) rotates each half-row cyclically left by r while
+    sigma_(2N-1) swaps the rows (the point of the order)."""
+    d = Pm.c1_params()
+    oc = orc.Ctx(d["log_n"], d["q"], d["p"], d["t"], d["gamma"], d["plain"])
+    n, row = oc.n, oc.n // 2
+    mp = oc.seal_index_map()
+    assert sorted(int(x) for x in mp) == list(range(n))
+    rng = np.random.default_rng(5)
+    v = rng.integers(0, oc.plain, size=n, dtype=np.uint64)
+    m = oc.encode_seal(v)
+    assert np.array_equal(oc.decode_seal(m), v)
+    for r in (1, 3):
+        g = oc.galois_elt(r)
+        rot = oc.decode_seal(oc.automorphism(g, m[None], oc.oPlain, 1)[0])
+        want = np.concatenate([np.roll(v[:row], -r), np.roll(v[row:], -r)])
+        assert np.array_equal(rot, want), r
+    conj = oc.decode_seal(oc.automorphism(2 * n - 1, m[None], oc.oPlain, 1)[0])
+    assert np.array_equal(conj, np.concatenate([v[row:], v[:row]]))
