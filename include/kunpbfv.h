@@ -209,6 +209,18 @@ int kbfv_mult_relin_graph_create(const kbfv_context *ctx, const uint64_t *rlk, c
 int kbfv_graph_launch(kbfv_graph *graph, void *stream);
 void kbfv_graph_destroy(kbfv_graph *graph);
 
+/* Ciphertext arithmetic over [batch][2][qn][N], NTT form (BFVScheme::negate,
+ * multConstAndEqual(uint64), multConstAndAddCipherEqual, addConstAndEqual,
+ * include/BFVScheme.cuh:192-326).  out may alias the inputs.
+ * kbfv_mul_scalar: out = k * ct (+ add when add is not null).
+ * kbfv_add_plain:  out = ct + (0, NTT(Delta m)), m: [batch][N] encoded plaintexts. */
+int kbfv_negate(const kbfv_context *ctx, const uint64_t *ct, uint64_t *out, int batch, void *stream);
+int kbfv_mul_scalar(const kbfv_context *ctx, const uint64_t *ct, uint64_t k, const uint64_t *add,
+                    uint64_t *out, int batch, void *stream);
+size_t kbfv_add_plain_workspace_bytes(const kbfv_context *ctx, int batch);
+int kbfv_add_plain(const kbfv_context *ctx, const uint64_t *ct, const uint64_t *m, uint64_t *out, int batch,
+                   void *workspace, size_t ws_bytes, void *stream);
+
 /* Batch encoder (BFVContext::encode / decode, include/Encoder.cuh:23-113):
  * INTT / NTT over t when t = 1 mod 2N, identity (coefficient encoding) otherwise.
  * In place on [batch][N]. */

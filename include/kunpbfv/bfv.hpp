@@ -150,6 +150,11 @@ class BFVScheme {
     void addAndEqual(Ciphertext &cipher1, Ciphertext &cipher2);
     void sub(Ciphertext &cipher_res, Ciphertext &cipher1, Ciphertext &cipher2);
     void subAndEqual(Ciphertext &cipher1, Ciphertext &cipher2);
+    void negate(Ciphertext &cipher_res, Ciphertext &cipher);
+    void negateAndEqual(Ciphertext &cipher);
+    // plaintext (from encode) or constant added to every slot
+    void addConstAndEqual(Ciphertext &cipher, Plaintext &cnst);
+    void addConstAndEqual(Ciphertext &cipher, uint64_tt cnst);
 
     void mult_23(Ciphertext &cipher_res, Ciphertext &cipher1, Ciphertext &cipher2);
     void multAndEqual_23(Ciphertext &cipher1, Ciphertext &cipher2);
@@ -157,6 +162,10 @@ class BFVScheme {
     void multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector<Ciphertext *> &c2);
     // plaintext (from encode_ntt) times ciphertext (include/BFVScheme.cuh:226)
     void multConstAndEqual(Ciphertext &cipher, Plaintext &cnst);
+    void multConstAndEqual(Ciphertext &cipher, uint64_tt cnst);
+    void multConstAndAddCipherEqual(Ciphertext &c1, Ciphertext &c2, uint64_tt cnst);  // c1 += c2 * cnst
+    void square(Ciphertext &cipher_res, Ciphertext &cipher);
+    void squareAndEqual(Ciphertext &cipher);
 
     // automorphisms with key switching (include/KeySwitch_23.cuh:466-584)
     void leftRotateAndEqual_23(Ciphertext &cipher, long rotSlots);

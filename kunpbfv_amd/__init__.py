@@ -61,6 +61,10 @@ SIGNATURES = {
     "kbfv_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_negate": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "kbfv_mul_scalar": (C.c_int, [_vp, _vp, _u64, _vp, _vp, C.c_int, _vp]),
+    "kbfv_add_plain_workspace_bytes": (_sz, [_vp, C.c_int]),
+    "kbfv_add_plain": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
     "kbfv_encode_seal": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_decode_seal": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_galois_elt": (_u64, [_vp, _i64]),
@@ -354,6 +358,24 @@ class Context:
         ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
         check(lib.kbfv_mult_relin(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
                                   stream_ptr(stream)))
+        return out
+
+    def negate(self, ct, out=None, stream=None):
+        out = out if out is not None else self.empty(*ct.shape)
+        check(lib.kbfv_negate(self.h, ptr(ct), ptr(out), ct.shape[0], stream_ptr(stream)))
+        return out
+
+    def mul_scalar(self, ct, k, add=None, out=None, stream=None):
+        out = out if out is not None else self.empty(*ct.shape)
+        a = ptr(add) if add is not None else None
+        check(lib.kbfv_mul_scalar(self.h, ptr(ct), k, a, ptr(out), ct.shape[0], stream_ptr(stream)))
+        return out
+
+    def add_plain(self, ct, m, out=None, stream=None):
+        out = out if out is not None else self.empty(*ct.shape)
+        ws = self.workspace(lib.kbfv_add_plain_workspace_bytes(self.h, ct.shape[0]))
+        check(lib.kbfv_add_plain(self.h, ptr(ct), ptr(m), ptr(out), ct.shape[0], ptr(ws), ws.numel(),
+                                 stream_ptr(stream)))
         return out
 
     def encode_seal(self, vals, stream=None):

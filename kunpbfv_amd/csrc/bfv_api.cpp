@@ -363,6 +363,39 @@ void BFVScheme::sub(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
 }
 void BFVScheme::subAndEqual(Ciphertext &c1, Ciphertext &c2) { sub(c1, c1, c2); }
 
+// include/BFVScheme.cuh:192-326
+void BFVScheme::negate(Ciphertext &res, Ciphertext &cipher) {
+    res.l = cipher.l;
+    ok(kbfv_negate(context.handle, (uint64_t *)cipher.cipher_device, (uint64_t *)res.cipher_device, 1,
+                   context.stream),
+       "negate");
+}
+void BFVScheme::negateAndEqual(Ciphertext &cipher) { negate(cipher, cipher); }
+
+void BFVScheme::addConstAndEqual(Ciphertext &cipher, Plaintext &cnst) {
+    cipher.l = std::min(cipher.l, cnst.l);
+    void *w = ensure_ws(kbfv_add_plain_workspace_bytes(context.handle, 1));
+    ok(kbfv_add_plain(context.handle, (uint64_t *)cipher.cipher_device, (uint64_t *)cnst.mx_device,
+                      (uint64_t *)cipher.cipher_device, 1, w, ws_bytes, context.stream),
+       "addConst");
+    hip_ok(hipStreamSynchronize(context.stream), "addConst");
+}
+
+void BFVScheme::addConstAndEqual(Ciphertext &cipher, uint64_tt cnst) {
+    // the constant polynomial cnst: every slot (batching) or the constant term
+    std::vector<uint64_t> m((size_t)context.N, 0);
+    m[0] = cnst % context.plain_modulus;
+    uint64_t *dm = nullptr;
+    hip_ok(hipMalloc(&dm, m.size() * 8), "hipMalloc");
+    hip_ok(hipMemcpy(dm, m.data(), m.size() * 8, hipMemcpyHostToDevice), "copy");
+    void *w = ensure_ws(kbfv_add_plain_workspace_bytes(context.handle, 1));
+    const int rc = kbfv_add_plain(context.handle, (uint64_t *)cipher.cipher_device, dm,
+                                  (uint64_t *)cipher.cipher_device, 1, w, ws_bytes, context.stream);
+    (void)hipStreamSynchronize(context.stream);
+    (void)hipFree(dm);
+    ok(rc, "addConst");
+}
+
 void BFVScheme::mult_23(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
     if (c1.l == 0 || c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
     if (!rlk_23) throw std::invalid_argument("rotKey_23 not exists");
@@ -399,6 +432,26 @@ void BFVScheme::multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector
     dfree(a);
     dfree(b);
 }
+
+void BFVScheme::multConstAndEqual(Ciphertext &cipher, uint64_tt cnst) {
+    if (cnst == 1) return;
+    if (cipher.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    ok(kbfv_mul_scalar(context.handle, (uint64_t *)cipher.cipher_device, cnst, nullptr,
+                       (uint64_t *)cipher.cipher_device, 1, context.stream),
+       "multConst");
+}
+
+void BFVScheme::multConstAndAddCipherEqual(Ciphertext &c1, Ciphertext &c2, uint64_tt cnst) {
+    if (c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    c1.l = std::min(c1.l, c2.l);
+    ok(kbfv_mul_scalar(context.handle, (uint64_t *)c2.cipher_device, cnst, (uint64_t *)c1.cipher_device,
+                       (uint64_t *)c1.cipher_device, 1, context.stream),
+       "multConstAndAddCipher");
+}
+
+// include/KeySwitch_23.cuh:375-466 (no secret-key shortcut there either)
+void BFVScheme::square(Ciphertext &res, Ciphertext &cipher) { mult_23(res, cipher, cipher); }
+void BFVScheme::squareAndEqual(Ciphertext &cipher) { mult_23(cipher, cipher, cipher); }
 
 void BFVScheme::multConstAndEqual(Ciphertext &cipher, Plaintext &cnst) {
     if (cipher.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");

@@ -122,6 +122,31 @@ __global__ void k_ksk_combine(kb_dev_tables t, const uint64_t *__restrict__ e,
     b[i] = kb_submod(v, kb_mulmod(a[i], key_pq[r], q, mlo, mhi), q);
 }
 
+// ciphertext negation over [batch][2][qn][N]
+__global__ void k_negate(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out, int oQ,
+                         int qn, int n, long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t q = t.mod[oQ + (int)((i / n) % qn)];
+    const uint64_t x = in[i];
+    out[i] = x ? q - x : 0;
+}
+
+// out = k * in (+ add) over [batch][2][qn][N]; k reduced per limb by Barrett
+template <bool ADD>
+__global__ void k_mul_scalar(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t kval,
+                             const uint64_t *__restrict__ add, uint64_t *__restrict__ out, int oQ, int qn, int n,
+                             long total) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int m = oQ + (int)((i / n) % qn);
+    const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+    const uint64_t kq = kb_barrett128(kbfv_u128{kval, 0}, q, mlo, mhi);
+    uint64_t v = kb_mulmod(in[i], kq, q, mlo, mhi);
+    if (ADD) v = kb_addmod(v, add[i], q);
+    out[i] = v;
+}
+
 // slot permutation over [batch][N]: SCATTER out[map[i]] = in[i], else out[i] = in[map[i]]
 template <bool SCATTER>
 __global__ void k_slot_permute(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
@@ -629,6 +654,49 @@ extern "C" int kbfv_encode(const kbfv_context *c, uint64_t *vals, int batch, voi
     if (!c->batching) return KBFV_OK;  // coefficient encoding
     return kb_ntt_inplace(c, -1, vals, c->oPlain, 1, batch, S(st));
 }
+// ---------------------------------------------------------------- ciphertext arithmetic
+// (BFVScheme::negate / multConstAndEqual / multConstAndAddCipherEqual /
+// addConstAndEqual, include/BFVScheme.cuh:192-326)
+extern "C" int kbfv_negate(const kbfv_context *c, const uint64_t *ct, uint64_t *out, int batch, void *st) {
+    KB_REQUIRE(c && ct && out && batch > 0, "null argument");
+    const long total = (long)batch * 2 * c->qn * c->n;
+    hipLaunchKernelGGL(k_negate, dim3(blocks_for(total, BS)), dim3(BS), 0, S(st), c->t, ct, out, c->oQ, c->qn,
+                       c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+extern "C" int kbfv_mul_scalar(const kbfv_context *c, const uint64_t *ct, uint64_t k, const uint64_t *add,
+                               uint64_t *out, int batch, void *st) {
+    KB_REQUIRE(c && ct && out && batch > 0, "null argument");
+    const long total = (long)batch * 2 * c->qn * c->n;
+    if (add)
+        hipLaunchKernelGGL(k_mul_scalar<true>, dim3(blocks_for(total, BS)), dim3(BS), 0, S(st), c->t, ct, k, add,
+                           out, c->oQ, c->qn, c->n, total);
+    else
+        hipLaunchKernelGGL(k_mul_scalar<false>, dim3(blocks_for(total, BS)), dim3(BS), 0, S(st), c->t, ct, k,
+                           nullptr, out, c->oQ, c->qn, c->n, total);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+extern "C" size_t kbfv_add_plain_workspace_bytes(const kbfv_context *c, int batch) {
+    return (c && batch > 0) ? (size_t)batch * c->qn * c->n * sizeof(uint64_t) : 0;
+}
+extern "C" int kbfv_add_plain(const kbfv_context *c, const uint64_t *ct, const uint64_t *m, uint64_t *out,
+                              int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && ct && m && out && ws && batch > 0, "null argument");
+    KB_REQUIRE(ws_bytes >= kbfv_add_plain_workspace_bytes(c, batch), "workspace too small");
+    const long N = c->n, L = (long)c->qn * N;
+    uint64_t *dm = (uint64_t *)ws;
+    hipStream_t s = S(st);
+    KB_TRY(kb_delta_m(c, m, dm, batch, s));
+    KB_TRY(kb_ntt_inplace(c, +1, dm, c->oQ, c->qn, batch, s));
+    for (int b = 0; b < batch; ++b) {  // a stays, b += NTT(Delta m)
+        if (out != ct) KB_HIP(hipMemcpyAsync(out + 2 * L * b, ct + 2 * L * b, L * 8, hipMemcpyDeviceToDevice, s));
+        KB_TRY(kb_poly_binop(c, 0, ct + 2 * L * b + L, dm + L * b, out + 2 * L * b + L, c->oQ, c->qn, 1, s));
+    }
+    return KBFV_OK;
+}
+
 // SEAL slot order (BFVContext::preComputeIndex, include/BFVcontext.cuh:102-129;
 // encode_BFV_kernel / decode_BFV_kernel, include/Encoder.cuh:6-21)
 extern "C" int kbfv_encode_seal(const kbfv_context *c, uint64_t *vals, uint64_t *tmp, int batch, void *st) {

@@ -304,6 +304,42 @@ void or_mult_relin(const or_ctx *c, const uint64_t *rlk_t, const uint64_t *ct1,
     free(ks);
 }
 
+void or_negate(const or_ctx *c, const uint64_t *ct, uint64_t *out) {
+    size_t N = c->n;
+    for (int h = 0; h < 2; h++)
+        for (int i = 0; i < c->qn; i++) {
+            uint64_t q = c->mod[c->oQ + i];
+            for (size_t k = 0; k < N; k++) {
+                size_t o = ((size_t)h * c->qn + i) * N + k;
+                out[o] = ct[o] ? q - ct[o] : 0;
+            }
+        }
+}
+
+void or_mul_scalar(const or_ctx *c, const uint64_t *ct, uint64_t kval, const uint64_t *add, uint64_t *out) {
+    size_t N = c->n;
+    for (int h = 0; h < 2; h++)
+        for (int i = 0; i < c->qn; i++) {
+            uint64_t q = c->mod[c->oQ + i], kq = kval % q;
+            for (size_t k = 0; k < N; k++) {
+                size_t o = ((size_t)h * c->qn + i) * N + k;
+                uint64_t v = or_mulmod(ct[o], kq, q);
+                if (add) v = (v + add[o]) % q;
+                out[o] = v;
+            }
+        }
+}
+
+void or_add_plain(const or_ctx *c, const uint64_t *ct, const uint64_t *m, uint64_t *out) {
+    size_t N = c->n, L = (size_t)c->qn * N;
+    uint64_t *dm = alloc0(L);
+    or_add_delta_m(c, m, dm);
+    or_ntt_fwd_range(c, dm, c->oQ, c->qn, 1);
+    memcpy(out, ct, L * 8);
+    pw_add(c, ct + L, dm, out + L, c->oQ, c->qn);
+    free(dm);
+}
+
 /* batching: slots -> coefficients by INTT over t (include/Encoder.cuh:23-62);
  * without a 2N-th root of unity mod t this is coefficient encoding */
 void or_encode(const or_ctx *c, const uint64_t *slots, uint64_t *m) {

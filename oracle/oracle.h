@@ -133,6 +133,11 @@ void or_mult_tensor(const or_ctx *c, const uint64_t *ct1, const uint64_t *ct2, u
 void or_relin(const or_ctx *c, const uint64_t *rlk_t, const uint64_t *d2_coeff, uint64_t *ks_out);
 void or_add(const or_ctx *c, const uint64_t *a, const uint64_t *b, uint64_t *out, int limbs,
             int mod0, int poly_num);
+/* ciphertext arithmetic over Q, NTT form (include/BFVScheme.cuh:192-326):
+ * out = -ct; out = c * ct (+ add); out = ct + (0, NTT(Delta m)) */
+void or_negate(const or_ctx *c, const uint64_t *ct, uint64_t *out);
+void or_mul_scalar(const or_ctx *c, const uint64_t *ct, uint64_t k, const uint64_t *add, uint64_t *out);
+void or_add_plain(const or_ctx *c, const uint64_t *ct, const uint64_t *m, uint64_t *out);
 void or_encode(const or_ctx *c, const uint64_t *slots, uint64_t *m);
 void or_decode(const or_ctx *c, const uint64_t *m, uint64_t *slots);
 /* SEAL slot order (BFVContext::preComputeIndex, include/BFVcontext.cuh:102-129):

@@ -79,6 +79,9 @@ def _load():
         "or_rgsw_encrypt": (None, [C.c_void_p, u8p, u64p, u64p, C.c_int, u64p]),
         "or_external_product": (None, [C.c_void_p, u64p, u64p, C.c_int, u64p]),
         "or_galois_elt": (C.c_uint64, [C.c_void_p, C.c_long]),
+        "or_negate": (None, [C.c_void_p, u64p, u64p]),
+        "or_mul_scalar": (None, [C.c_void_p, u64p, C.c_uint64, u64p, u64p]),
+        "or_add_plain": (None, [C.c_void_p, u64p, u64p, u64p]),
         "or_seal_index_map": (None, [C.c_void_p, u64p]),
         "or_encode_seal": (None, [C.c_void_p, u64p, u64p]),
         "or_decode_seal": (None, [C.c_void_p, u64p, u64p]),
@@ -274,6 +277,19 @@ class Ctx:
         out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
         lib.or_external_product(self.h, P(rg), P(c), base_bits, P(out))
         return out
+
+    def negate(self, ct):
+        return self._call(lib.or_negate, [ct], (2, self.qn, self.n))
+
+    def mul_scalar(self, ct, k, add=None):
+        c = u64(ct)
+        out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
+        a = u64(add) if add is not None else None
+        lib.or_mul_scalar(self.h, P(c), k, P(a) if a is not None else None, P(out))
+        return out
+
+    def add_plain(self, ct, m):
+        return self._call(lib.or_add_plain, [ct, m], (2, self.qn, self.n))
 
     def seal_index_map(self):
         out = np.zeros(self.n, dtype=np.uint64)

@@ -136,3 +136,22 @@ def test_reference_logn15_gamma4_is_rejected():
     with pytest.raises(ValueError):
         orc.Ctx(15, d["q"], d["p"], d["t"], 4, d["plain"])
     orc.Ctx(15, d["q"], d["p"], d["t"], 2, d["plain"])
+
+
+def test_ciphertext_arithmetic_decrypts(keys):
+    """negate, scalar multiply (with accumulate) and plaintext addition decrypt
+    to -m, k*m (+ m2) and m + m2 (reference BFVScheme.cuh:192-326)."""
+    oc, sk, pk, _ = keys
+    rng = np.random.default_rng(9)
+    t = oc.plain
+    m1 = rng.integers(0, t, size=oc.n, dtype=np.uint64)
+    m2 = rng.integers(0, t, size=oc.n, dtype=np.uint64)
+    c1 = oc.encrypt(bytes([11]) * 32, pk, oc.encode(m1))
+    c2 = oc.encrypt(bytes([12]) * 32, pk, oc.encode(m2))
+    dec = lambda ct: oc.decode(oc.decrypt(sk, ct)).astype(object)  # noqa: E731
+    a1, a2 = m1.astype(object), m2.astype(object)
+    assert np.array_equal(dec(oc.negate(c1)), (-a1) % t)
+    k = 12345
+    assert np.array_equal(dec(oc.mul_scalar(c1, k)), (k * a1) % t)
+    assert np.array_equal(dec(oc.mul_scalar(c1, 7, add=c2)), (7 * a1 + a2) % t)
+    assert np.array_equal(dec(oc.add_plain(c1, oc.encode(m2))), (a1 + a2) % t)
