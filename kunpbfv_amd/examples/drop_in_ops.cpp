@@ -1,0 +1,121 @@
+// The rest of the reference's BFVScheme surface on the C++ drop-in API:
+// left rotation, conjugation, negation, constant multiply / add, plaintext add,
+// square.  Each result is decrypted and checked slot by slot on the host.
+// Slots are in the reference's order (NTT index j is the evaluation at
+// psi^(2 brv(j) + 1)), so the automorphism X -> X^g moves slot j to the slot
+// whose point is psi^((2 brv(j) + 1) g).
+// Usage: drop_in_ops [N]   (default 16384)
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <vector>
+
+#include "kunpbfv/bfv.hpp"
+
+static uint64_tt brv(uint64_tt x, int bits) {
+    uint64_tt r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+int main(int argc, char *argv[]) {
+    size_t n_arg = argc > 1 ? strtoull(argv[1], nullptr, 10) : 16384;
+    BFVContext context(n_arg);
+    BFVScheme scheme(context);
+    SecretKey sk(context);
+    scheme.mallocMemory();
+    scheme.addEncKey(sk);
+    scheme.addMultKey_23(sk);
+    scheme.addLeftRotKey_23(sk, 1);
+    scheme.addConjKey_23(sk);
+
+    const int N = (int)context.N, L = context.L, slots = context.slots;
+    const uint64_tt t = context.plain_modulus;
+    int logn = 0;
+    while ((1 << logn) < N) ++logn;
+    std::vector<uint64_tt> m1(slots), m2(slots);
+    srand(3);
+    for (int i = 0; i < slots; ++i) m1[i] = rand() % t, m2[i] = rand() % t;
+
+    uint64_tt *d1, *d2, *dd;
+    (void)hipMalloc(&d1, 8 * slots);
+    (void)hipMalloc(&d2, 8 * slots);
+    (void)hipMalloc(&dd, 8 * slots);
+    auto up = [&](uint64_tt *d, const std::vector<uint64_tt> &h) {
+        (void)hipMemcpy(d, h.data(), 8 * slots, hipMemcpyHostToDevice);
+    };
+    Plaintext p1(N, L, L, slots), p2(N, L, L, slots), pd(N, L, L, slots);
+    Ciphertext c(N, L, L, slots), c2(N, L, L, slots), r(N, L, L, slots);
+    up(d1, m1);
+    up(d2, m2);
+    context.encode(d1, p1);
+    context.encode(d2, p2);
+    auto fresh = [&]() { scheme.encryptMsg(c, p1); };
+    auto check = [&](const char *what, Ciphertext &ct, const std::vector<uint64_tt> &want) {
+        scheme.decryptMsg(pd, sk, ct);
+        context.decode(pd, dd);
+        std::vector<uint64_tt> got(slots);
+        (void)hipMemcpy(got.data(), dd, 8 * slots, hipMemcpyDeviceToHost);
+        int bad = 0;
+        for (int i = 0; i < slots; ++i) bad += got[i] != want[i];
+        printf("%-22s slots_wrong=%d\n", what, bad);
+        return bad;
+    };
+    // slot permutation of X -> X^g in the reference's slot order
+    auto perm = [&](uint64_tt g) {
+        std::map<uint64_tt, int> at;
+        const uint64_tt m2n = 2 * (uint64_tt)N;
+        for (int j = 0; j < N; ++j) at[(2 * brv(j, logn) + 1) % m2n] = j;
+        std::vector<uint64_tt> w(slots);
+        for (int j = 0; j < N; ++j) w[j] = m1[at[(2 * brv(j, logn) + 1) * g % m2n]];
+        return w;
+    };
+    int bad = 0;
+    std::vector<uint64_tt> w(slots);
+
+    fresh();
+    scheme.leftRotateAndEqual_23(c, 1);
+    uint64_tt g5 = 5;
+    bad += check("leftRotateAndEqual_23", c, perm(g5));
+
+    fresh();
+    scheme.conjugate_23(r, c);
+    bad += check("conjugate_23", r, perm(2 * (uint64_tt)N - 1));
+
+    fresh();
+    scheme.negateAndEqual(c);
+    for (int i = 0; i < slots; ++i) w[i] = (t - m1[i]) % t;
+    bad += check("negateAndEqual", c, w);
+
+    fresh();
+    scheme.multConstAndEqual(c, (uint64_tt)12345);
+    for (int i = 0; i < slots; ++i) w[i] = (uint64_tt)((unsigned __int128)m1[i] * 12345 % t);
+    bad += check("multConstAndEqual", c, w);
+
+    fresh();
+    scheme.addConstAndEqual(c, (uint64_tt)777);
+    for (int i = 0; i < slots; ++i) w[i] = (m1[i] + 777) % t;
+    bad += check("addConstAndEqual(u64)", c, w);
+
+    fresh();
+    scheme.addConstAndEqual(c, p2);
+    for (int i = 0; i < slots; ++i) w[i] = (m1[i] + m2[i]) % t;
+    bad += check("addConstAndEqual(pt)", c, w);
+
+    fresh();
+    scheme.encryptMsg(c2, p2);
+    scheme.multConstAndAddCipherEqual(c, c2, 3);
+    for (int i = 0; i < slots; ++i) w[i] = (uint64_tt)((m1[i] + (unsigned __int128)3 * m2[i]) % t);
+    bad += check("multConstAndAddCipher", c, w);
+
+    fresh();
+    scheme.squareAndEqual(c);
+    for (int i = 0; i < slots; ++i) w[i] = (uint64_tt)((unsigned __int128)m1[i] * m1[i] % t);
+    bad += check("squareAndEqual", c, w);
+
+    printf("N=%d total_wrong=%d\n", N, bad);
+    (void)hipFree(d1);
+    (void)hipFree(d2);
+    (void)hipFree(dd);
+    return bad == 0 ? 0 : 1;
+}

@@ -21,3 +21,18 @@ def test_correct_mult_driver(n, extra):
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "slots_wrong=0" in r.stdout
+
+
+OPS = os.path.join(ROOT, "kunpbfv_amd", "examples", "drop_in_ops")
+
+
+@pytest.mark.parametrize("n", [4096, 16384])
+def test_drop_in_ops_driver(n):
+    """rotation, conjugation, negation, constant multiply/add, plaintext add and
+    square through the reference's BFVScheme member names"""
+    assert os.path.exists(OPS), "example driver not built (run __graft_entry__.build())"
+    r = subprocess.run([OPS, str(n)], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "total_wrong=0" in r.stdout
+
