@@ -164,14 +164,22 @@ __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restr
     __syncthreads();
 }
 
-template <int LOG_S, bool COLUMN, bool FWD>
+// pass sizes: the column pass takes ceil(log_n / 2) levels, the row pass the rest
+template <int LOG_N, bool COLUMN>
+constexpr int pass_log() { return COLUMN ? (LOG_N + 1) / 2 : LOG_N - (LOG_N + 1) / 2; }
+
+template <int LOG_N, bool COLUMN, bool FWD>
 __global__ void __launch_bounds__(TILE >> LOG_R)
     k_ntt_pass(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
                const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
-               const uint64_t *__restrict__ mods, int log_n, int mod0, int limbs, int polys,
-               int batch, int groups, long sB, long sY, long sL, long dB, long dY, long dL) {
+               const uint64_t *__restrict__ mods, int mod0, int limbs, int polys,
+               int batch, long sB, long sY, long sL, long dB, long dY, long dL) {
+    constexpr int LOG_S = pass_log<LOG_N, COLUMN>();
+    constexpr int log_n = LOG_N;
     using G = Geo<LOG_S, COLUMN>;
     constexpr int S = G::S, C = G::C, T = G::T;
+    constexpr int groups = (1 << (LOG_N - LOG_S)) / C;  // workgroups per limb
+    static_assert(groups >= 1, "ring too small for the tile");
     constexpr bool LAST = (FWD != COLUMN);  // forward ends with the row pass
     __shared__ uint64_t tile[G::WORDS];
 
@@ -183,14 +191,14 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     const int bi = bid % batch;
     const int l = bid / batch;
 
-    const int n = 1 << log_n;
+    constexpr int n = 1 << log_n;
     const int m = mod0 + l;
     const uint64_t q = mods[m];
     const kbfv_tw *tab = tw + ((size_t)m * 2 + (FWD ? 0 : 1)) * n;
     const uint64_t *in = src + bi * sB + y * sY + l * sL;
     uint64_t *out = dst + bi * dB + y * dY + l * dL;
-    const int stride = n >> LOG_S;
-    const int g0 = COLUMN ? 0 : (log_n - LOG_S);
+    constexpr int stride = n >> LOG_S;
+    constexpr int g0 = COLUMN ? 0 : (log_n - LOG_S);
 
     constexpr int LR0 = LOG_S < LOG_R ? LOG_S : LOG_R;
     constexpr int LR1 = LOG_S - LR0;
@@ -237,41 +245,38 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     }
 }
 
-template <int LOG_S, bool COLUMN, bool FWD>
+template <int LOG_N, bool COLUMN, bool FWD>
 int launch_pass(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
                 int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
                 hipStream_t st) {
-    using G = Geo<LOG_S, COLUMN>;
-    const int subs = c->n >> LOG_S;  // columns (column pass) or chunks (row pass)
-    KB_REQUIRE(subs >= G::C, "ring degree 2^%d too small for the NTT tiling", c->log_n);
-    const int groups = subs / G::C;
-    const long blocks = (long)groups * limbs * polys * batch;
+    using G = Geo<pass_log<LOG_N, COLUMN>(), COLUMN>;
+    const long groups = (1L << (LOG_N - pass_log<LOG_N, COLUMN>())) / G::C;
+    const long blocks = groups * limbs * polys * batch;
     KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
-    hipLaunchKernelGGL((k_ntt_pass<LOG_S, COLUMN, FWD>), dim3((unsigned)blocks), dim3(G::T), 0, st,
-                       src, dst, c->t.tw, c->t.ninv, c->t.mod, c->log_n, mod0, limbs, polys, batch,
-                       groups, sB, sY, sL, dB, dY, dL);
+    hipLaunchKernelGGL((k_ntt_pass<LOG_N, COLUMN, FWD>), dim3((unsigned)blocks), dim3(G::T), 0, st, src, dst,
+                       c->t.tw, c->t.ninv, c->t.mod, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL);
     KB_LAUNCH_CHECK();
     return KBFV_OK;
 }
 
 template <bool COLUMN, bool FWD>
-int launch_pass_s(int log_s, const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0,
-                  int limbs, int polys, int batch, long sB, long sY, long sL, long dB, long dY,
-                  long dL, hipStream_t st) {
-    switch (log_s) {
-        case 6:
-            return launch_pass<6, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB,
-                                               dY, dL, st);
-        case 7:
-            return launch_pass<7, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB,
-                                               dY, dL, st);
-        case 8:
-            return launch_pass<8, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB,
-                                               dY, dL, st);
+int launch_pass_n(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs, int polys,
+                  int batch, long sB, long sY, long sL, long dB, long dY, long dL, hipStream_t st) {
+#define KB_NTT_N(LN)                                                                                     \
+    case LN:                                                                                             \
+        return launch_pass<LN, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, \
+                                            dL, st);
+    switch (c->log_n) {
+        KB_NTT_N(12)
+        KB_NTT_N(13)
+        KB_NTT_N(14)
+        KB_NTT_N(15)
+        KB_NTT_N(16)
         default:
-            kb_set_error("unsupported NTT pass size 2^%d", log_s);
+            kb_set_error("unsupported ring degree 2^%d", c->log_n);
             return KBFV_EINVAL;
     }
+#undef KB_NTT_N
 }
 
 }  // namespace
@@ -284,20 +289,15 @@ int kb_ntt(const kbfv_context *c, int dir, const uint64_t *src, uint64_t *dst, i
     KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod, "modulus range out of bounds");
     KB_REQUIRE(polys > 0 && batch > 0, "empty NTT");
     if (mod0 + limbs - 1 == c->oPlain) KB_REQUIRE(c->batching, "plain modulus has no 2N-th root");
-    // column pass takes ceil(log_n / 2) levels, row pass the rest
-    const int lc = (c->log_n + 1) / 2, lr = c->log_n - lc;
     int rc;
     if (dir > 0) {
-        rc = launch_pass_s<true, true>(lc, c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY,
-                                       dL, st);
+        rc = launch_pass_n<true, true>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
         if (rc) return rc;
-        return launch_pass_s<false, true>(lr, c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB,
-                                          dY, dL, st);
+        return launch_pass_n<false, true>(c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, st);
     }
-    rc = launch_pass_s<false, false>(lr, c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY,
-                                     dL, st);
+    rc = launch_pass_n<false, false>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
     if (rc) return rc;
-    return launch_pass_s<true, false>(lc, c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY,
+    return launch_pass_n<true, false>(c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY,
                                       dL, st);
 }
 
