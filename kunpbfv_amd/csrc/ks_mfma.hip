@@ -85,7 +85,9 @@ __global__ void __launch_bounds__(IM_THREADS, 2)
             const int pos = e % IM_POS;
             const int d = (e / IM_POS) % IM_KP;
             const int ct = tile * IM_CT + e / (IM_POS * IM_KP);
-            pre[r] = (d < D && ct < batch) ? cin[(((long)ct * D + d) * tn + kk) * N + n0 + pos] : 0;
+            const bool live = d < D && ct < batch;  // branch-free: clamp, load, then select
+            const uint64_t v = cin[(((long)(live ? ct : 0) * D + (live ? d : 0)) * tn + kk) * N + n0 + pos];
+            pre[r] = live ? v : 0;
         }
     };
     fetch(0);

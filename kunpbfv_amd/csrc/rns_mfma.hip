@@ -57,27 +57,43 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
     uint64_t *yp = y + poly * out_stride + n0;
 
     // inputs: lane (position p16 of tile mt, group g) holds K rows 8c + 2g + e
+    // all loads first (rows past in_n read row 0 and are zeroed afterwards), so
+    // that every load is in flight before the first use
     uint64_t yv[RG_MT][KC][2];
+    kbfv_tw hv[KC][2];
+    uint64_t qv[KC][2];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
+    for (int c = 0; c < KC; ++c)
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
             const int i = 8 * c + 2 * g + e;
-            kbfv_tw h{0, 0};
-            uint64_t qi = 0;
-            if (!SR && i < in_n) h = hatinv[i], qi = mods_in[i];
+            const int ii = i < in_n ? i : 0;
+#pragma unroll
+            for (int mt = 0; mt < RG_MT; ++mt) yv[mt][c][e] = xp[(long)ii * n + 16 * mt + p16];
+            if (!SR) hv[c][e] = hatinv[ii], qv[c][e] = mods_in[ii];
+        }
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const bool live = 8 * c + 2 * g + e < in_n;
 #pragma unroll
             for (int mt = 0; mt < RG_MT; ++mt) {
-                uint64_t v = 0;
-                if (i < in_n) {
-                    v = xp[(long)i * n + 16 * mt + p16];
-                    if (!SR) v = kb_shoup(v, h.w, h.ws, qi);
-                    ys[wave][i][16 * mt + p16] = v;
-                }
-                yv[mt][c][e] = v;
+                uint64_t v = yv[mt][c][e];
+                if (!SR) v = kb_shoup(v, hv[c][e].w, hv[c][e].ws, qv[c][e]);
+                yv[mt][c][e] = live ? v : 0;
             }
         }
-    }
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int i = 8 * c + 2 * g + e;
+            if (i < in_n) {
+#pragma unroll
+                for (int mt = 0; mt < RG_MT; ++mt) ys[wave][i][16 * mt + p16] = yv[mt][c][e];
+            }
+        }
     __syncthreads();
     // the reference's double estimate, one position per lane, in input order
     {

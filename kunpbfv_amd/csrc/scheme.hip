@@ -641,7 +641,10 @@ extern "C" int kbfv_rotate(const kbfv_context *c, const uint64_t *gkey, uint64_t
         KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * nb, 1, 0, (long)c->tn * N, N, 0,
                       (long)c->tn * N, N, s));
         KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, nb, s));
-        KB_TRY(kb_moddown(c, w.PQ, w.MD, w.X + qn * N, CTS, nb, s));
+        KB_TRY(kb_moddown(c, w.PQ, w.MD, nullptr, 0, nb, s));
+        for (int b = 0; b < nb; ++b)  // KS(a) + (0, b): b joins the second polynomial only
+            KB_TRY(kb_poly_binop(c, 0, w.MD + b * CTS + qn * N, w.X + b * CTS + qn * N, w.MD + b * CTS + qn * N,
+                                 c->oQ, qn, 1, s));
         KB_TRY(launch_automorphism(c, g, w.MD, w.X, c->oQ, qn, 2 * nb, s));
         KB_TRY(kb_ntt(c, +1, w.X, out + b0 * CTS, c->oQ, qn, 2, nb, CTS, qn * N, N, CTS, qn * N, N, s));
     }
