@@ -511,9 +511,15 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     // (2^(8u) M_ij mod p_j); then per j the 128-bit offset p_j*ceil(2^76/p_j)
     // that makes the signed digit sum non-negative.  Padded to 16 ceil(J/16)
     // rows and 8 ceil(K/8) columns with zeros.
-    auto mfma_table = [&](int K, int J, const std::vector<uint64_t> &pj, auto entry) {
+    // Expanded matrix [JP][8][KP] of packed digit bytes, then per output row j
+    // KB_MX_ROW words: the offset (-KB_MX_ACC_BIAS * sum_v 2^(8v)) mod p_j that
+    // cancels the accumulator bias, 2^64 - p_j, the offset and 1/p_j as doubles,
+    // and the diagonal coefficient of scale-and-round with its Shoup companion.
+    auto mfma_table = [&](int K, int J, const std::vector<uint64_t> &pj, auto entry, auto diag) {
         const int KP = 8 * ((K + 7) / 8), JP = 16 * ((J + 15) / 16);
-        std::vector<uint64_t> v((size_t)JP * 8 * KP + 2 * (size_t)JP, 0);
+        std::vector<uint64_t> v((size_t)JP * 8 * KP + KB_MX_ROW * (size_t)JP, 0);
+        kb_u128_host bias_sum = 0;
+        for (int u = 0; u < 8; u++) bias_sum += (kb_u128_host)KB_MX_ACC_BIAS << (8 * u);
         for (int j = 0; j < J; j++) {
             for (int i = 0; i < K; i++) {
                 const uint64_t m = entry(i, j) % pj[j];
@@ -524,23 +530,35 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
                         v[((size_t)j * 8 + w) * KP + i] |= ((d >> (8 * w)) & 0xff) << (8 * u);
                 }
             }
-            const kb_u128_host two76 = (kb_u128_host)1 << 76;
-            const kb_u128_host off = ((two76 + pj[j] - 1) / pj[j]) * pj[j];
-            v[(size_t)JP * 8 * KP + 2 * j] = (uint64_t)off;
-            v[(size_t)JP * 8 * KP + 2 * j + 1] = (uint64_t)(off >> 64);
+            uint64_t *row = v.data() + (size_t)JP * 8 * KP + KB_MX_ROW * (size_t)j;
+            const uint64_t off = (pj[j] - (uint64_t)(bias_sum % pj[j])) % pj[j];
+            const double offd = (double)off, invp = 1.0 / (double)pj[j];
+            const uint64_t dg = diag(j) % pj[j];
+            row[0] = off;
+            row[1] = 0 - pj[j];
+            std::memcpy(&row[2], &offd, 8);
+            std::memcpy(&row[3], &invp, 8);
+            row[4] = dg;
+            row[5] = shoup(dg, pj[j]);
         }
         return v;
     };
-    const bool mx_ok = rn + 1 <= 32;  // every matrix of the MFMA path has K, J <= 32
+    // every matrix of the MFMA path has K, J <= 32, and its epilogue's quotient
+    // estimate needs output primes of at least 2^58
+    bool mx_ok = rn + 1 <= 32;
+    for (int i = 0; i < qn + rn; i++) mx_ok = mx_ok && mod[c->oQ + i] >= (1ULL << 58);  // Q then R
     std::vector<uint64_t> mx_qr, mx_rq, mx_sr;
     if (mx_ok) {
-        mx_qr = mfma_table(qn + 1, rn, R, [&](int i, int j) {
-            return i < qn ? q_hat_mod_r[j * qn + i] : (R[j] - alphaQ[rn + j]) % R[j];
-        });
-        mx_rq = mfma_table(rn + 1, qn, Q, [&](int i, int j) {
-            return i < rn ? r_hat_mod_q[j * rn + i] : (Q[j] - alphaR[qn + j]) % Q[j];
-        });
-        mx_sr = mfma_table(qn, rn, R, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; });
+        auto none = [](int) { return (uint64_t)0; };
+        mx_qr = mfma_table(
+            qn + 1, rn, R,
+            [&](int i, int j) { return i < qn ? q_hat_mod_r[j * qn + i] : (R[j] - alphaQ[rn + j]) % R[j]; }, none);
+        mx_rq = mfma_table(
+            rn + 1, qn, Q,
+            [&](int i, int j) { return i < rn ? r_hat_mod_q[j * rn + i] : (Q[j] - alphaR[qn + j]) % Q[j]; }, none);
+        mx_sr = mfma_table(
+            qn, rn, R, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; },
+            [&](int j) { return sr_modr[j * (qn + 1) + qn]; });
     }
 
     // SEAL slot order (include/BFVcontext.cuh:102-129): row 0 slot i at the NTT

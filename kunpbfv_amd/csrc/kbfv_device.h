@@ -92,12 +92,24 @@ __device__ __forceinline__ uint64_t kb_submod(uint64_t a, uint64_t b, uint64_t q
 __device__ __forceinline__ uint64_t kb_f64_to_u64_sat(double x) {
     return x >= 18446744073709551616.0 ? ~0ULL : (uint64_t)x;
 }
+// uint64 -> double with one rounding (error at most 2^11)
+__device__ __forceinline__ double kb_u64_to_f64(uint64_t x) {
+    return fma((double)(uint32_t)(x >> 32), 4294967296.0, (double)(uint32_t)x);
+}
 
 // Shoup lazy product from 32-bit halves: r = x*w - floor(x*ws / 2^64) * q in
 // [0, 2q) for any x < 2^64, with nq = 2^64 - q so that both low products
 // accumulate in one 64-bit chain (x*w + h*nq = x*w - h*q mod 2^64).
 __device__ __forceinline__ uint64_t kb_mad32(uint32_t a, uint32_t b, uint64_t c) {
     return (uint64_t)a * b + c;
+}
+// a*b mod 2^32 as one opaque v_mul_lo_u32.  Written as plain C, the compiler
+// folds the cross products below back into 64x64-bit multiplies and expands
+// those again with extra v_mad_u64_u32 steps (about twice the instructions).
+__device__ __forceinline__ uint32_t kb_mul_lo32(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_lo_u32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 __device__ __forceinline__ uint64_t kb_shoup_lazy_nq(uint64_t x, uint64_t w, uint64_t ws, uint64_t nq) {
     const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
@@ -109,7 +121,8 @@ __device__ __forceinline__ uint64_t kb_shoup_lazy_nq(uint64_t x, uint64_t w, uin
     const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
     const uint32_t n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
     const uint64_t r = kb_mad32(h0, n0, kb_mad32(x0, w0, 0));
-    const uint32_t cross = x0 * w1 + x1 * w0 + h0 * n1 + h1 * n0;
+    const uint32_t cross = kb_mul_lo32(x0, w1) + kb_mul_lo32(x1, w0) + kb_mul_lo32(h0, n1) +
+                           kb_mul_lo32(h1, n0);
     return r + ((uint64_t)cross << 32);
 }
 // a >= b ? a - b : a, for a, b < 2^62 (the sign of the difference decides)

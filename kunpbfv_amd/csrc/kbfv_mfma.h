@@ -21,8 +21,14 @@
 //     sum_k x_k m_k = sum_(k,u) d_ku (2^(8u) m_k mod p)   (mod p),
 // so K runs over (k, u) with the data digits as they are and the matrix side
 // holds the balanced digits v = 0..7 of the precomputed 2^(8u) m_k mod p: 8
-// MFMA passes (one per v) instead of 15, no zero bytes, and the signed sum
-// (|.| < K 2^70) is made non-negative with a per-row offset, a multiple of p.
+// MFMA passes (one per v) instead of 15 and no zero bytes.  With biased
+// accumulators A_v in [0, 2^23] (KB_MX_ACC_BIAS) the sum is
+//     V = sum_v A_v 2^(8v) + off,   off = (-bias sum_v 2^(8v)) mod p,
+// a non-negative integer below 2^80 congruent to the dot product; the four
+// 32-bit words A_(2k+1) 2^8 + A_2k give V exactly mod 2^64 and, in double
+// precision, a quotient estimate that is one of floor(V/p) - 1, floor(V/p)
+// for p >= 2^58, so one multiply-subtract and one conditional subtraction
+// finish the reduction.
 //
 // MFMA v_mfma_i32_16x16x64_i8 lane layout (checked on the device): lane l holds
 // row (A) / column (B) l % 16 and K bytes 16 (l / 16) .. 16 (l / 16) + 15; the
@@ -34,6 +40,14 @@
 typedef int kb_v4i __attribute__((ext_vector_type(4)));
 
 #define KB_DIGIT_BIAS 0x8080808080808080ULL
+
+// Expanded-matrix products start each digit pass's int32 accumulator at this
+// bias: a pass sums at most 256 products of two digits in [-128, 127], so the
+// biased sums lie in [0, 2^23] and pairs of them combine as unsigned 32-bit
+// words.  The table's per-row offset cancels the bias modulo p.
+#define KB_MX_ACC_BIAS (1 << 22)
+// words per output row after the expanded matrix
+#define KB_MX_ROW 6
 
 __device__ __forceinline__ uint64_t kb_digits(uint64_t x) { return (x + KB_DIGIT_BIAS) ^ KB_DIGIT_BIAS; }
 

@@ -31,6 +31,8 @@ constexpr int RG_MT = 4;                 // 16-position tiles per wave
 constexpr int RG_POS = 16 * RG_MT;       // positions per wave (one per lane in the estimate)
 constexpr int RG_WAVES = 4;              // waves per workgroup
 constexpr int RG_BLOCK = RG_POS * RG_WAVES;
+constexpr int RG_KMAX = 32;              // K rows of the largest matrix
+static_assert(RG_KMAX * RG_POS * 2 >= 4 * 4 * RG_MT * 64, "W words must fit the wave's input tile");
 
 // KC: 8-row chunks of K (K <= 8 KC);  NJ: 16-row tiles of output limbs.
 // SR = false: base conversion (Shoup-scaled inputs, alpha row at K index in_n);
@@ -41,14 +43,15 @@ template <int KC, int NJ, bool SR>
 __global__ void __launch_bounds__(64 * RG_WAVES, 2)
     k_rns_gemm(const uint64_t *__restrict__ x, long in_stride, uint64_t *__restrict__ y, long out_stride,
                const uint64_t *__restrict__ mat, const uint64_t *__restrict__ mods_out,
-               const uint64_t *__restrict__ mus_out, const kbfv_tw *__restrict__ hatinv,
-               const uint64_t *__restrict__ mods_in, const double *__restrict__ est,
-               const uint64_t *__restrict__ sr_rows, int in_n, int out_n, int n) {
+               const kbfv_tw *__restrict__ hatinv, const uint64_t *__restrict__ mods_in,
+               const double *__restrict__ est, int in_n, int out_n, int n) {
     constexpr int KP = 8 * KC, JP = 16 * NJ;
-    __shared__ uint64_t ys[RG_WAVES][KP][RG_POS];
+    // ys: the wave's inputs for the estimate, then (reused) its 32-bit words W
+    __shared__ uint64_t ys[RG_WAVES][RG_KMAX][RG_POS];
     __shared__ uint64_t alpha_s[RG_WAVES][RG_POS];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    uint32_t *const wl = (uint32_t *)&ys[wave][0][0];  // W words [k][i][mt][lane]
     const int p16 = lane & 15, g = lane >> 4;
     const int tiles = n / RG_BLOCK;
     const long poly = blockIdx.x / tiles;
@@ -117,76 +120,101 @@ __global__ void __launch_bounds__(64 * RG_WAVES, 2)
             bop[mt][c] = kb_pack2(kb_digits(d0), kb_digits(d1));
         }
     }
-    const uint64_t *offs = mat + (long)JP * 8 * KP;
-    // one 16-limb output tile at a time: 8 digit passes of the expanded matrix,
-    // Horner in 2^8 into two int64 halves, then offset, reduction and stores
+    const uint64_t *rows = mat + (long)JP * 8 * KP;
+    // one 16-limb output tile at a time: 8 digit passes of the expanded matrix
+    // from biased accumulators, paired into the 32-bit words W3..W0, then the
+    // double-precision quotient estimate, one multiply-subtract and stores
 #pragma unroll 1
     for (int nt = 0; nt < NJ; ++nt) {
         const kb_v4i *mrow = (const kb_v4i *)(mat + (long)(16 * nt + p16) * 8 * KP + 8 * 0 + 2 * g);
-        int64_t Shi[RG_MT][4], Slo[RG_MT][4];
+        // W[k] = A_(2k+1) 2^8 + A_2k goes to LDS as soon as its pair of passes is
+        // done, which keeps the pass loop's registers to the operands
+        uint32_t W[RG_MT][4];
+        // E[j][v][8c + 2g .. +1]: 16 bytes, row stride 8 KP words, v stride KP
+        // words; the next pass's operands load while this pass multiplies, and
+        // the scheduling fences keep deeper prefetch from exhausting registers
+        kb_v4i mcur[KC], mnxt[KC];
 #pragma unroll
-        for (int mt = 0; mt < RG_MT; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) Shi[mt][i] = 0, Slo[mt][i] = 0;
+        for (int c = 0; c < KC; ++c) mcur[c] = mrow[(7 * KP + 8 * c) / 2];
 #pragma unroll
         for (int v = 7; v >= 0; --v) {
+            if (v > 0) {
+#pragma unroll
+                for (int c = 0; c < KC; ++c) mnxt[c] = mrow[((v - 1) * KP + 8 * c) / 2];
+            }
             kb_v4i acc[RG_MT];
 #pragma unroll
-            for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_v4i{0, 0, 0, 0};
+            for (int mt = 0; mt < RG_MT; ++mt)
+                acc[mt] = kb_v4i{KB_MX_ACC_BIAS, KB_MX_ACC_BIAS, KB_MX_ACC_BIAS, KB_MX_ACC_BIAS};
 #pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                // E[j][v][8c + 2g .. +1]: 16 bytes, row stride 8 KP words, v stride KP words
-                const kb_v4i a = mrow[(v * KP + 8 * c) / 2];
+            for (int c = 0; c < KC; ++c)
 #pragma unroll
-                for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_mfma_i8(a, bop[mt][c], acc[mt]);
-            }
+                for (int mt = 0; mt < RG_MT; ++mt) acc[mt] = kb_mfma_i8(mcur[c], bop[mt][c], acc[mt]);
 #pragma unroll
             for (int mt = 0; mt < RG_MT; ++mt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    if (v >= 4)
-                        Shi[mt][i] = (int64_t)((uint64_t)Shi[mt][i] << 8) + (int64_t)acc[mt][i];
-                    else
-                        Slo[mt][i] = (int64_t)((uint64_t)Slo[mt][i] << 8) + (int64_t)acc[mt][i];
+                    if (v & 1) {
+                        W[mt][i] = (uint32_t)acc[mt][i];
+                    } else {
+                        W[mt][i] = (W[mt][i] << 8) + (uint32_t)acc[mt][i];
+                        wl[(((v >> 1) * 4 + i) * RG_MT + mt) * 64 + lane] = W[mt][i];
+                    }
                 }
+#pragma unroll
+            for (int c = 0; c < KC; ++c) mcur[c] = mnxt[c];
+            __builtin_amdgcn_sched_barrier(0);
         }
+        // the words come back from LDS below, not from registers
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int j = 16 * nt + 4 * g + i;
             if (j >= out_n) continue;
-            const uint64_t pj = mods_out[j], mlo = mus_out[2 * j], mhi = mus_out[2 * j + 1];
-            const uint64_t olo = offs[2 * j], ohi = offs[2 * j + 1];
+            const uint64_t *rw = rows + KB_MX_ROW * j;
+            const uint64_t pj = mods_out[j], off = rw[0], np = rw[1];
+            const double offd = __longlong_as_double((long long)rw[2]);
+            const double invp = __longlong_as_double((long long)rw[3]);
 #pragma unroll
             for (int mt = 0; mt < RG_MT; ++mt) {
-                // Shi 2^32 + Slo + offset (signed halves, non-negative total < 2^78)
-                uint64_t lo = (uint64_t)Shi[mt][i] << 32;
-                uint64_t hi = (uint64_t)(Shi[mt][i] >> 32);
-                const uint64_t sl = (uint64_t)Slo[mt][i];
-                uint64_t t = lo + sl;
-                hi += (uint64_t)(t < lo) + (Slo[mt][i] < 0 ? ~0ULL : 0ULL);
-                lo = t + olo;
-                hi += (uint64_t)(lo < t) + ohi;
-                kbfv_u128 vv{lo, hi};
+                const uint32_t w3 = wl[((3 * 4 + i) * RG_MT + mt) * 64 + lane];
+                const uint32_t w2 = wl[((2 * 4 + i) * RG_MT + mt) * 64 + lane];
+                const uint32_t w1 = wl[((1 * 4 + i) * RG_MT + mt) * 64 + lane];
+                const uint32_t w0 = wl[((0 * 4 + i) * RG_MT + mt) * 64 + lane];
+                // V = w3 2^48 + w2 2^32 + w1 2^16 + w0 + off: exactly mod 2^64, and in double
+                uint64_t lo = (uint64_t)w0 + off + ((uint64_t)w1 << 16) +
+                              ((uint64_t)(w2 + (w3 << 16)) << 32);
+                double vd = fma(fma(fma((double)w3, 65536.0, (double)w2), 65536.0, (double)w1), 65536.0,
+                                (double)w0) + offd;
                 if (SR) {
-                    kb_mac128(vv, xp[(long)(in_n + j) * n + 16 * mt + p16], sr_rows[(long)j * (in_n + 1) + in_n]);
-                    kb_add128(vv, alpha_s[wave][16 * mt + p16]);
+                    // diagonal term x_(qn+j) c_j (Shoup, [0, 2p)) and the rounded estimate
+                    const uint64_t d = kb_shoup_lazy_nq(xp[(long)(in_n + j) * n + 16 * mt + p16], rw[4], rw[5], np);
+                    const uint64_t al = alpha_s[wave][16 * mt + p16];
+                    lo += d + al;
+                    vd = vd + kb_u64_to_f64(d) + kb_u64_to_f64(al);
                 }
-                yp[(long)j * n + 16 * mt + p16] = kb_barrett128(vv, pj, mlo, mhi);
+                // |vd - V| < 2^30 and V < 2^80, so for p >= 2^58 the estimate minus
+                // 2^-20 truncates to floor(V/p) or one less: r = V - qe p in [0, 2p)
+                const uint32_t qe = (uint32_t)fmax(fma(vd, invp, -0x1p-20), 0.0);
+                const uint64_t r = kb_mad32(qe, (uint32_t)np, lo) +
+                                   ((uint64_t)kb_mul_lo32(qe, (uint32_t)(np >> 32)) << 32);
+                yp[(long)j * n + 16 * mt + p16] = kb_csub(r, pj);
             }
         }
+        // the next tile overwrites the words
+        asm volatile("" ::: "memory");
     }
 }
 
 template <bool SR>
 int launch_rns_gemm(int kc, int nj, dim3 grid, hipStream_t st, const uint64_t *x, long in_stride,
                     uint64_t *y, long out_stride, const uint64_t *mat, const uint64_t *mods_out,
-                    const uint64_t *mus_out, const kbfv_tw *hatinv, const uint64_t *mods_in,
-                    const double *est, const uint64_t *sr_rows, int in_n, int out_n, int n) {
+                    const kbfv_tw *hatinv, const uint64_t *mods_in, const double *est, int in_n,
+                    int out_n, int n) {
 #define KB_RG(KC, NJ)                                                                                 \
     if (kc == KC && nj == NJ) {                                                                        \
         hipLaunchKernelGGL((k_rns_gemm<KC, NJ, SR>), grid, dim3(64 * RG_WAVES), 0, st, x, in_stride, y, \
-                           out_stride, mat, mods_out, mus_out, hatinv, mods_in, est, sr_rows, in_n,     \
-                           out_n, n);                                                                   \
+                           out_stride, mat, mods_out, hatinv, mods_in, est, in_n, out_n, n);           \
         return 0;                                                                                      \
     }
     KB_RG(1, 1)
@@ -215,8 +243,8 @@ int kb_bconv_mfma(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in
     const int in0 = q_to_r ? c->oQ : c->oR, out0 = q_to_r ? c->oR : c->oQ;
     const dim3 grid((unsigned)((long)polys * (c->n / RG_BLOCK)));
     if (launch_rns_gemm<false>(kc, nj, grid, st, x, in_stride, y, out_stride, q_to_r ? t.mx_qr : t.mx_rq,
-                               t.mod + out0, t.mu + 2 * out0, q_to_r ? t.q_hatinv : t.r_hatinv, t.mod + in0,
-                               q_to_r ? t.q_inv : t.r_inv, nullptr, in_n, out_n, c->n))
+                               t.mod + out0, q_to_r ? t.q_hatinv : t.r_hatinv, t.mod + in0,
+                               q_to_r ? t.q_inv : t.r_inv, in_n, out_n, c->n))
         return 1;
     KB_LAUNCH_CHECK();
     return KBFV_OK;
@@ -227,8 +255,7 @@ int kb_scale_round_mfma(const kbfv_context *c, const uint64_t *x, uint64_t *y, i
     const kb_dev_tables &t = c->t;
     const dim3 grid((unsigned)((long)polys * (c->n / RG_BLOCK)));
     if (launch_rns_gemm<true>(kc, nj, grid, st, x, (long)(c->qn + c->rn) * c->n, y, (long)c->rn * c->n,
-                              t.mx_sr, t.mod + c->oR, t.mu + 2 * c->oR, nullptr, nullptr, t.sr_frac,
-                              t.sr_modr, c->qn, c->rn, c->n))
+                              t.mx_sr, t.mod + c->oR, nullptr, nullptr, t.sr_frac, c->qn, c->rn, c->n))
         return 1;
     KB_LAUNCH_CHECK();
     return KBFV_OK;
