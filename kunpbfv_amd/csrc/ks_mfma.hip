@@ -8,166 +8,189 @@
 // kbfv_mfma.h and one Barrett reduction per output, so the result equals the
 // scalar kernel's bit for bit.
 //
-// A workgroup takes 8 consecutive coefficients of one T limb, one wave per
-// coefficient.  The key of those coefficients is staged through LDS once and
-// kept in registers as byte-reversed digits; digit tiles of 32 ciphertexts are
-// then streamed through LDS (coalesced 64-byte rows in, transposed so a lane
-// reads its 16 operand bytes at once) and the outputs leave through LDS the
-// same way.  The key is read from HBM once per call, the digits and outputs
-// once each.
+// One wave per coefficient, no LDS and no barriers: a workgroup's 16 waves take
+// 16 consecutive coefficients of one T limb, so each 8-byte gather of a wave
+// shares its 128-byte line with the other 15 waves (the line is read from HBM
+// once).  A wave keeps its coefficient's key rows as byte-reversed digits
+// (kbfv_mfma.h, shifted per weight pass) and streams the ciphertexts through
+// in tiles of 16: 15 weight passes of MFMAs into biased accumulators (every
+// P_w + 2^22 lies in [0, 2^23]), paired into 32-bit words on the fly, then per
+// output the 128-bit sum rebuilt from its 8 words, the bias removed, and one
+// Barrett reduction.
 #include "kbfv_internal.h"
 #include "kbfv_mfma.h"
 #include "kbfv_ops.h"
 
 namespace {
 
-constexpr int IM_POS = 8;                    // coefficients per workgroup (one per wave)
-constexpr int IM_THREADS = 64 * IM_POS;      // 512
-constexpr int IM_CT = 32;                    // ciphertexts per tile (2 MFMA column tiles)
-constexpr int IM_ROWS = 32;                  // key rows (h, b) held: 2 x ri_blocks <= 32
-constexpr int IM_KP = 32;                    // K (digits) held: dnum <= 32
-constexpr int IM_RS = IM_KP + 2;             // LDS row stride: 16 lanes' 16-byte reads hit distinct banks
-constexpr int IM_PS = IM_ROWS * IM_RS + 2;   // LDS stride between coefficients, [pos][row][d]
-constexpr int IM_CS = IM_ROWS * IM_POS + 1;  // LDS stride between ciphertexts, [ct][row][pos]
-constexpr int IM_LDS = (IM_POS * IM_PS > IM_CT * IM_CS) ? IM_POS * IM_PS : IM_CT * IM_CS;
-// per-thread share of a digit tile (IM_CT x IM_KP x IM_POS values)
-constexpr int IM_PER = IM_CT * IM_KP * IM_POS / IM_THREADS;  // 16
+constexpr int IM_WAVES = 16;              // coefficients per workgroup (one per wave)
+constexpr int IM_THREADS = 64 * IM_WAVES;  // 1024
+constexpr int IM_CT = 16;                 // ciphertexts per tile (one MFMA column tile)
+// output staging [r][ct][pos] in LDS, double-buffered: pos stride 1, ct stride
+// 17 and row stride 276 (= 4 mod 8) make both the per-wave writes and the
+// row-wise reads bank-conflict free (8-byte banks, 32 lanes per cycle)
+constexpr int IM_OC = 17, IM_OR = 16 * IM_OC + 4, IM_OBUF = 32 * IM_OR;
+
+// Raw buffer access: a resource (base in scalar registers) plus a 32-bit byte
+// offset per lane, so an address costs one VGPR.  Every offset used here stays
+// inside its allocation by construction (clamped indices); num_records is left
+// at its maximum.
+typedef unsigned int kb_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t kb_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ uint64_t kb_bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    const kb_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+    return ((uint64_t)v.y << 32) | v.x;
+}
+__device__ __forceinline__ void kb_bstore(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, uint64_t x) {
+    kb_u32x2 v;
+    v.x = (uint32_t)x;
+    v.y = (uint32_t)(x >> 32);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, 0);
+}
+
+// sum_{w < 15} KB_MX_ACC_BIAS 2^(8w) mod 2^128: what the biased accumulators add
+constexpr unsigned __int128 ip_bias_sum() {
+    unsigned __int128 s = 0;
+    for (int w = 0; w < 15; ++w) s += (unsigned __int128)KB_MX_ACC_BIAS << (8 * w);
+    return s;
+}
+constexpr uint64_t IM_BIAS_LO = (uint64_t)ip_bias_sum();
+constexpr uint64_t IM_BIAS_HI = (uint64_t)(ip_bias_sum() >> 64);
 
 // KC: 8-digit chunks (dnum <= 8 KC); NJ: 16-row tiles of the 2 x ri_blocks key rows
 template <int KC, int NJ>
-__global__ void __launch_bounds__(IM_THREADS, 2)
+__global__ void __launch_bounds__(IM_THREADS, 4)
     k_ip_mfma(const uint64_t *__restrict__ cin, const uint64_t *__restrict__ key, uint64_t *__restrict__ out,
               const uint64_t *__restrict__ mods, const uint64_t *__restrict__ mus, int D, int B, int tn,
               int oT, int n, int batch) {
-    __shared__ uint64_t lds[IM_LDS];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;  // = this wave's coefficient within the tile
+    const int lane = threadIdx.x & 63;
     const int p16 = lane & 15, g = lane >> 4;
-    const int tiles = n / IM_POS;
-    const int kk = blockIdx.x / tiles;
-    const int n0 = (blockIdx.x % tiles) * IM_POS;
-    const long N = n;
+    const int groups = n / IM_WAVES;
+    const int kk = blockIdx.x / groups;
+    // this wave's coefficient (wave-uniform, so the base pointers below live in
+    // scalar registers and every lane address is a 32-bit offset)
+    const int pos = (blockIdx.x % groups) * IM_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long N = n, TN = (long)tn * n;
+    const uint32_t tn32 = (uint32_t)TN;  // 8 TN * 16 * 32 fits 31 bits (checked by the launcher)
     const int R2 = 2 * B;
     const uint64_t q = mods[oT + kk], mlo = mus[2 * (oT + kk)], mhi = mus[2 * (oT + kk) + 1];
 
-    // ---- key of the 8 coefficients -> LDS [pos][row][d] -> registers (reversed digits)
-    // key layout [2][D][B][tn][N]: row r = h*B + b
-    for (int e = tid; e < R2 * D * IM_POS; e += IM_THREADS) {
-        const int pos = e % IM_POS;
-        const int rest = e / IM_POS;
-        const int b = rest % B;
-        const int hd = rest / B;  // h*D + d
-        const int d = hd % D, h = hd / D;
-        const uint64_t v = key[((long)hd * B + b) * tn * N + (long)kk * N + n0 + pos];
-        lds[pos * IM_PS + (h * B + b) * IM_RS + d] = v;
-    }
-    __syncthreads();
+    // key layout [2][D][B][tn][N], row r = h B + b; lane holds rows 16 nt + p16,
+    // digits d = 8c + 2g + e (zero outside 2B x D)
     uint64_t rk[NJ][KC][2];
+    {
+        const uint64_t *kb = key + kk * N + pos;
 #pragma unroll
-    for (int nt = 0; nt < NJ; ++nt)
+        for (int nt = 0; nt < NJ; ++nt)
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int r = 16 * nt + p16, d = 8 * c + 2 * g + e;
+                    const bool live = r < R2 && d < D;
+                    const int rr = live ? r : 0, dd = live ? d : 0;
+                    const int h = rr / B, b = rr - h * B;
+                    const uint64_t v = kb[(size_t)(uint32_t)((h * D + dd) * B + b) * TN];
+                    rk[nt][c][e] = __builtin_bswap64(kb_digits(live ? v : 0));
+                }
+    }
+
+    // ciphertext digits, cin layout [batch][D][tn][N]: lane holds ciphertext
+    // 16 tile + p16, digits d = 8c + 2g + e
+    const int ntiles = (batch + IM_CT - 1) / IM_CT;
+    __shared__ uint64_t obuf[2][IM_OBUF];
+    const int wave = pos % IM_WAVES;
+    uint64_t xin[KC][2];
+    // rows d >= D meet zero key digits and columns past the batch are not
+    // stored, so those lanes only need some valid address (clamped), not zeros
+    auto fetch = [&](int tile) {
+        const __amdgpu_buffer_rsrc_t xb = kb_rsrc(cin + (long)tile * IM_CT * D * TN + kk * N + pos);
+        // lane offsets are recomputed per tile (hoisted, they would hold registers)
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const int p16 = ln & 15, g = ln >> 4;
+        const int cl = min(p16, batch - 1 - tile * IM_CT);
 #pragma unroll
         for (int c = 0; c < KC; ++c)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                const int r = 16 * nt + p16, d = 8 * c + 2 * g + e;
-                const uint64_t v = (r < R2 && d < D) ? lds[wave * IM_PS + r * IM_RS + d] : 0;
-                rk[nt][c][e] = __builtin_bswap64(kb_digits(v));
+                const int d = min(8 * c + 2 * g + e, D - 1);
+                xin[c][e] = kb_bload(xb, (uint32_t)(cl * D + d) * tn32 * 8u);
             }
-
-    // ---- digit tiles: cin layout [batch][D][tn][N]
-    const int ntiles = (batch + IM_CT - 1) / IM_CT;
-    uint64_t pre[IM_PER];
-    auto fetch = [&](int tile) {
-#pragma unroll
-        for (int r = 0; r < IM_PER; ++r) {
-            const int e = tid + r * IM_THREADS;  // (ct, d, pos), pos fastest
-            const int pos = e % IM_POS;
-            const int d = (e / IM_POS) % IM_KP;
-            const int ct = tile * IM_CT + e / (IM_POS * IM_KP);
-            const bool live = d < D && ct < batch;  // branch-free: clamp, load, then select
-            const uint64_t v = cin[(((long)(live ? ct : 0) * D + (live ? d : 0)) * tn + kk) * N + n0 + pos];
-            pre[r] = live ? v : 0;
-        }
     };
-    fetch(0);
     for (int tile = 0; tile < ntiles; ++tile) {
-        __syncthreads();  // the previous tile's outputs have left LDS
+        // no prefetch of the next tile: its registers would spill; the other
+        // waves of the SIMD cover the load latency
+        fetch(tile);
+        kb_v4i bop[KC];
 #pragma unroll
-        for (int r = 0; r < IM_PER; ++r) {
-            const int e = tid + r * IM_THREADS;
-            const int pos = e % IM_POS;
-            const int d = (e / IM_POS) % IM_KP;
-            const int ctl = e / (IM_POS * IM_KP);
-            lds[pos * IM_PS + ctl * IM_RS + d] = pre[r];
-        }
-        __syncthreads();
-        kb_v4i bop[2][KC];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                const uint64_t *src = lds + wave * IM_PS + (16 * mt + p16) * IM_RS + 8 * c + 2 * g;
-                bop[mt][c] = kb_pack2(kb_digits(src[0]), kb_digits(src[1]));
-            }
-        __syncthreads();  // the tile's digits are in registers; LDS now takes outputs
-        if (tile + 1 < ntiles) fetch(tile + 1);
+        for (int c = 0; c < KC; ++c) bop[c] = kb_pack2(kb_digits(xin[c][0]), kb_digits(xin[c][1]));
+        const int ct = tile * IM_CT + p16;  // result column
+        uint64_t *const ob = obuf[tile & 1];
 
-        // the key operands stay loop-carried: without this the compiler hoists
-        // all 15 shifted copies out of the tile loop and runs out of registers
-#pragma unroll
-        for (int nt = 0; nt < NJ; ++nt)
-#pragma unroll
-            for (int c = 0; c < KC; ++c) asm volatile("" : "+v"(rk[nt][c][0]), "+v"(rk[nt][c][1]));
 #pragma unroll
         for (int nt = 0; nt < NJ; ++nt) {
-            int64_t S[2][4];
-            uint64_t Hl[2][4], Hh[2][4];
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) S[mt][i] = 0, Hl[mt][i] = 0, Hh[mt][i] = 0;
+            // W[i][k] = P'_(2k+1) 2^8 + P'_2k (k < 7), W[i][7] = P'_14
+            uint32_t W[4][8];
 #pragma unroll
             for (int w = 14; w >= 0; --w) {
-                kb_v4i acc[2] = {kb_v4i{0, 0, 0, 0}, kb_v4i{0, 0, 0, 0}};
+                kb_v4i acc = kb_v4i{KB_MX_ACC_BIAS, KB_MX_ACC_BIAS, KB_MX_ACC_BIAS, KB_MX_ACC_BIAS};
 #pragma unroll
                 for (int c = 0; c < KC; ++c) {
+                    // shift here, not earlier: hoisted shifted copies exhaust registers
+                    asm volatile("" : "+v"(rk[nt][c][0]), "+v"(rk[nt][c][1]));
                     const kb_v4i a = kb_pack2(kb_weight_op(rk[nt][c][0], w), kb_weight_op(rk[nt][c][1], w));
-                    acc[0] = kb_mfma_i8(a, bop[0][c], acc[0]);
-                    acc[1] = kb_mfma_i8(a, bop[1][c], acc[1]);
+                    acc = kb_mfma_i8(a, bop[c], acc);
                 }
 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        S[mt][i] = (int64_t)((uint64_t)S[mt][i] << 8) + (int64_t)acc[mt][i];
-                        if (w == 12 || w == 6 || w == 0) {
-                            kb_horner48(Hl[mt][i], Hh[mt][i], S[mt][i]);
-                            S[mt][i] = 0;
-                        }
-                    }
+                for (int i = 0; i < 4; ++i) {
+                    if (w == 14 || (w & 1))
+                        W[i][w >> 1] = (uint32_t)acc[i];
+                    else
+                        W[i][w >> 1] = (W[i][w >> 1] << 8) + (uint32_t)acc[i];
+                    // pin the word here: sunk to the epilogue it would keep all 15
+                    // accumulators alive
+                    asm volatile("" : "+v"(W[i][w >> 1]));
+                }
                 // keep each weight pass together (no hoisting of later passes' operands)
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // result lane: ciphertext 16 mt + p16, key rows 16 nt + 4 g + i
+            // result lane: ciphertext ct, key rows 16 nt + 4 g + i
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int p16 = ln & 15, g = ln >> 4;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = 16 * nt + 4 * g + i;
-#pragma unroll
-                for (int mt = 0; mt < 2; ++mt)
-                    lds[(16 * mt + p16) * IM_CS + r * IM_POS + wave] =
-                        kb_barrett128(kbfv_u128{Hl[mt][i], Hh[mt][i]}, q, mlo, mhi);
+                // V' = sum_k W_k 2^(16k) mod 2^128, then V = V' - bias (exact: 0 <= V < 2^127)
+                const uint64_t a0 = (uint64_t)W[i][0] + ((uint64_t)W[i][1] << 16);
+                const uint64_t a1 = (uint64_t)W[i][2] + ((uint64_t)W[i][3] << 16);
+                const uint64_t a2 = (uint64_t)W[i][4] + ((uint64_t)W[i][5] << 16);
+                const uint64_t a3 = (uint64_t)W[i][6] + ((uint64_t)W[i][7] << 16);
+                const uint64_t lo = a0 + (a1 << 32);
+                uint64_t hi = (a1 >> 32) + a2 + (a3 << 32) + (uint64_t)(lo < a0);
+                hi -= IM_BIAS_HI + (uint64_t)(lo < IM_BIAS_LO);
+                const uint64_t v = kb_barrett128(kbfv_u128{lo - IM_BIAS_LO, hi}, q, mlo, mhi);
+                ob[r * IM_OR + p16 * IM_OC + wave] = v;
+                // one output at a time (interleaving them costs registers, not time)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
+        // the tile's outputs leave in 128-byte rows (16 coefficients); the other
+        // buffer takes the next tile, whose writes start after every wave has
+        // passed this barrier and hence finished reading that buffer's last tile
         __syncthreads();
-        // outputs [batch][2][B][tn][N]: row (ct, r) -> 8 consecutive coefficients
-        for (int e = tid; e < IM_CT * R2 * IM_POS; e += IM_THREADS) {
-            const int pos = e % IM_POS;
-            const int r = (e / IM_POS) % R2;
-            const int ctl = e / (IM_POS * R2);
-            const int ct = tile * IM_CT + ctl;
-            if (ct < batch)
-                out[(((long)ct * R2 + r) * tn + kk) * N + n0 + pos] = lds[ctl * IM_CS + r * IM_POS + pos];
+        {
+            int t = threadIdx.x;
+            asm volatile("" : "+v"(t));  // recomputed per tile rather than held
+            const int pp = t & 15, ctl = (t >> 4) & 15;
+            const int ctg = tile * IM_CT + ctl;
+            const __amdgpu_buffer_rsrc_t og = kb_rsrc(out + (long)tile * IM_CT * R2 * TN + kk * N + (pos - wave));
+            for (int r = t >> 8; r < R2; r += IM_THREADS / 256)
+                if (ctg < batch)
+                    kb_bstore(og, (uint32_t)(ctl * R2 + r) * tn32 * 8u + pp * 8u, ob[r * IM_OR + ctl * IM_OC + pp]);
         }
     }
 }
@@ -177,9 +200,10 @@ __global__ void __launch_bounds__(IM_THREADS, 2)
 int kb_inner_product_mfma(const kbfv_context *c, const uint64_t *cin, const uint64_t *key, uint64_t *out,
                           int batch, hipStream_t st) {
     const int D = c->dnum, B = c->ri_blocks;
-    if (D > IM_KP || 2 * B > IM_ROWS || c->n % IM_POS) return 1;
+    if (D > 32 || 2 * B > 32 || c->n % IM_WAVES) return 1;
+    if ((long)c->tn * c->n * 8 * IM_CT * 32 >= (1L << 31)) return 1;  // 32-bit byte offsets
     const int kc = (D + 7) / 8, nj = (2 * B + 15) / 16;
-    const dim3 grid((unsigned)(c->tn * (c->n / IM_POS)));
+    const dim3 grid((unsigned)(c->tn * (c->n / IM_WAVES)));
 #define KB_IM(KC, NJ)                                                                                   \
     if (kc == KC && nj == NJ) {                                                                          \
         hipLaunchKernelGGL((k_ip_mfma<KC, NJ>), grid, dim3(IM_THREADS), 0, st, cin, key, out, c->t.mod, \
