@@ -26,9 +26,9 @@ namespace {
 constexpr int IM_WAVES = 16;              // coefficients per workgroup (one per wave)
 constexpr int IM_THREADS = 64 * IM_WAVES;  // 1024
 constexpr int IM_CT = 16;                 // ciphertexts per tile (one MFMA column tile)
-// output staging [r][ct][pos] in LDS, double-buffered: pos stride 1, ct stride
-// 17 and row stride 276 (= 4 mod 8) make both the per-wave writes and the
-// row-wise reads bank-conflict free (8-byte banks, 32 lanes per cycle)
+// output staging [r][ct][pos] in LDS: pos stride 1, ct stride 17 and row
+// stride 276 (= 4 mod 8) make both the per-wave writes and the row-wise reads
+// bank-conflict free (8-byte banks, 32 lanes per cycle)
 constexpr int IM_OC = 17, IM_OR = 16 * IM_OC + 4, IM_OBUF = 32 * IM_OR;
 
 // Raw buffer access: a resource (base in scalar registers) plus a 32-bit byte
@@ -97,38 +97,43 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
                 }
     }
 
-    // ciphertext digits, cin layout [batch][D][tn][N]: lane holds ciphertext
-    // 16 tile + p16, digits d = 8c + 2g + e
+    // ciphertext digits, cin layout [batch][D][tn][N], staged per tile of 16
+    // ciphertexts through LDS as [ct][d][pos ^ ct]: 128-byte rows of 16
+    // coefficients come in, and the 16 lanes of a wave that read one pos (its
+    // coefficient) for 16 ciphertexts hit 16 different banks
     const int ntiles = (batch + IM_CT - 1) / IM_CT;
-    __shared__ uint64_t obuf[2][IM_OBUF];
+    __shared__ uint64_t ibuf[IM_CT * 32 * IM_WAVES];
+    __shared__ uint64_t obuf[IM_OBUF];
     const int wave = pos % IM_WAVES;
-    uint64_t xin[KC][2];
-    // rows d >= D meet zero key digits and columns past the batch are not
-    // stored, so those lanes only need some valid address (clamped), not zeros
-    auto fetch = [&](int tile) {
-        const __amdgpu_buffer_rsrc_t xb = kb_rsrc(cin + (long)tile * IM_CT * D * TN + kk * N + pos);
-        // lane offsets are recomputed per tile (hoisted, they would hold registers)
-        int ln = lane;
-        asm volatile("" : "+v"(ln));
-        const int p16 = ln & 15, g = ln >> 4;
-        const int cl = min(p16, batch - 1 - tile * IM_CT);
-#pragma unroll
-        for (int c = 0; c < KC; ++c)
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int d = min(8 * c + 2 * g + e, D - 1);
-                xin[c][e] = kb_bload(xb, (uint32_t)(cl * D + d) * tn32 * 8u);
-            }
-    };
     for (int tile = 0; tile < ntiles; ++tile) {
-        // no prefetch of the next tile: its registers would spill; the other
-        // waves of the SIMD cover the load latency
-        fetch(tile);
+        {
+            int t = threadIdx.x;
+            asm volatile("" : "+v"(t));  // recomputed per tile rather than held
+            const int pp = t & 15;
+            const __amdgpu_buffer_rsrc_t xb = kb_rsrc(cin + (long)tile * IM_CT * D * TN + kk * N + (pos - wave));
+            // rows j = 32 ct + d; rows d >= D stay unwritten (they meet zero key
+            // digits) and ciphertexts past the batch repeat the last one
+            for (int j = t >> 4; j < IM_CT * 32; j += IM_THREADS / 16) {
+                const int ctl = j >> 5, d = j & 31;
+                if (d < D) {
+                    const int cl = min(ctl, batch - 1 - tile * IM_CT);
+                    ibuf[j * 16 + (pp ^ ctl)] = kb_bload(xb, ((uint32_t)(cl * D + d) * tn32 + pp) * 8u);
+                }
+            }
+        }
+        __syncthreads();
         kb_v4i bop[KC];
+        {
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int p16 = ln & 15, g = ln >> 4;
 #pragma unroll
-        for (int c = 0; c < KC; ++c) bop[c] = kb_pack2(kb_digits(xin[c][0]), kb_digits(xin[c][1]));
-        const int ct = tile * IM_CT + p16;  // result column
-        uint64_t *const ob = obuf[tile & 1];
+            for (int c = 0; c < KC; ++c) {
+                const uint64_t *src = ibuf + (p16 * 32 + 8 * c + 2 * g) * 16 + (wave ^ p16);
+                bop[c] = kb_pack2(kb_digits(src[0]), kb_digits(src[16]));
+            }
+        }
+        uint64_t *const ob = obuf;
 
 #pragma unroll
         for (int nt = 0; nt < NJ; ++nt) {
@@ -178,9 +183,9 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // the tile's outputs leave in 128-byte rows (16 coefficients); the other
-        // buffer takes the next tile, whose writes start after every wave has
-        // passed this barrier and hence finished reading that buffer's last tile
+        // the tile's outputs leave in 128-byte rows (16 coefficients).  Both
+        // buffers are reused by the next tile only after its first barrier, by
+        // which every thread has finished reading them here and above
         __syncthreads();
         {
             int t = threadIdx.x;
