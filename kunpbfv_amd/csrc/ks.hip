@@ -189,15 +189,13 @@ __global__ void k_inner_product_generic(kb_dev_tables t, const uint64_t *__restr
 }
 
 // in [batch][2][B][tn][N] coefficient form -> out [batch][2][pq][N]
+// grid (n / KS_BS, B, batch * 2): coefficient, block b, ciphertext-poly (bt*2 + h)
 __global__ void k_modup_t_to_pq(kb_dev_tables t, const uint64_t *__restrict__ in,
                                 uint64_t *__restrict__ out, int pq, int tn, int gamma, int B, int oT,
-                                int n, long total) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int k = (int)(i % n);
-    long rest = i / n;
-    const int b = (int)(rest % B);
-    rest /= B;  // rest = bt*2 + h
+                                int n) {
+    const int k = blockIdx.x * KS_BS + threadIdx.x;
+    const int b = blockIdx.y;
+    const long rest = blockIdx.z;
     const uint64_t *ip = in + (rest * B + b) * (long)tn * n + k;
     uint64_t rb[KS_MAXG];
     float vv = 0;
@@ -224,17 +222,15 @@ __global__ void k_modup_t_to_pq(kb_dev_tables t, const uint64_t *__restrict__ in
 }
 
 // in [batch][2][pq][N] -> out [batch][2][qn][N] (+ add [batch](2 polys at add_stride))
+// grid (n / KS_BS, qn, batch * 2): coefficient, limb l, ciphertext-poly (bt*2 + h)
 __global__ void k_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                           const uint64_t *__restrict__ add, long add_stride, int pn, int qn, int oQ,
-                          int n, long total) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int k = (int)(i % n);
-    long rest = i / n;
-    const int l = (int)(rest % qn);
-    rest /= qn;  // bt*2 + h
-    const int h = (int)(rest % 2);
-    const long bt = rest / 2;
+                          int n) {
+    const int k = blockIdx.x * KS_BS + threadIdx.x;
+    const int l = blockIdx.y;
+    const long rest = blockIdx.z;
+    const int h = (int)(rest & 1);
+    const long bt = rest >> 1;
     const int pq = pn + qn;
     const uint64_t *ip = in + rest * (long)pq * n + k;
     const int m = oQ + l;
@@ -255,7 +251,7 @@ __global__ void k_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint
     const kbfv_tw pi = t.Pinv_mod_q[l];
     uint64_t v = kb_shoup(ra, pi.w, pi.ws, q);
     if (add) v = kb_addmod(v, add[bt * add_stride + ((long)h * qn + l) * n + k], q);
-    out[i] = v;
+    out[(rest * qn + l) * (long)n + k] = v;
 }
 
 }  // namespace
@@ -314,9 +310,9 @@ int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t 
 int kb_modup_t_to_pq(const kbfv_context *c, const uint64_t *in, uint64_t *out, int batch, hipStream_t st) {
     KB_REQUIRE(c && in && out && batch > 0, "bad modUp arguments");
     KB_REQUIRE(c->tn <= KS_MAXG, "too many T primes");
-    const long total = (long)batch * 2 * c->ri_blocks * c->n;
-    hipLaunchKernelGGL(k_modup_t_to_pq, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t, in, out,
-                       c->pn + c->qn, c->tn, c->gamma, c->ri_blocks, c->oT, c->n, total);
+    KB_REQUIRE(2L * batch <= 65535, "batch too large for one modUp launch");
+    hipLaunchKernelGGL(k_modup_t_to_pq, dim3(c->n / KS_BS, c->ri_blocks, 2 * batch), dim3(KS_BS), 0, st, c->t,
+                       in, out, c->pn + c->qn, c->tn, c->gamma, c->ri_blocks, c->oT, c->n);
     KB_LAUNCH_CHECK();
     return KBFV_OK;
 }
@@ -324,9 +320,9 @@ int kb_modup_t_to_pq(const kbfv_context *c, const uint64_t *in, uint64_t *out, i
 int kb_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
                long add_stride, int batch, hipStream_t st) {
     KB_REQUIRE(c && in && out && batch > 0, "bad modDown arguments");
-    const long total = (long)batch * 2 * c->qn * c->n;
-    hipLaunchKernelGGL(k_moddown, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t, in, out, add,
-                       add_stride, c->pn, c->qn, c->oQ, c->n, total);
+    KB_REQUIRE(2L * batch <= 65535, "batch too large for one modDown launch");
+    hipLaunchKernelGGL(k_moddown, dim3(c->n / KS_BS, c->qn, 2 * batch), dim3(KS_BS), 0, st, c->t, in, out, add,
+                       add_stride, c->pn, c->qn, c->oQ, c->n);
     KB_LAUNCH_CHECK();
     return KBFV_OK;
 }

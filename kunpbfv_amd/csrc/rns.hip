@@ -98,17 +98,15 @@ __global__ void __launch_bounds__(BC_TILE *BC_WAVES)
 // ------------------------------------------------------------------ tensor
 // Q part from the NTT-domain ciphertexts, R part from the extended polys.
 // ct: [batch][2][qn][N]; r: [batch][2][rn][N]; out: [batch][3][qn+rn][N]
+// grid (n / 256, qn + rn, batch): coefficient, limb l, ciphertext b
 __global__ void k_tensor_qr(kb_dev_tables t, const uint64_t *__restrict__ ct1,
                             const uint64_t *__restrict__ ct2, const uint64_t *__restrict__ r1,
                             const uint64_t *__restrict__ r2, uint64_t *__restrict__ out, int qn,
-                            int rn, int oQ, int n, long total) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
+                            int rn, int oQ, int n) {
     const int sn = qn + rn;
-    const int k = (int)(i % n);
-    const long rest = i / n;
-    const int l = (int)(rest % sn);
-    const long b = rest / sn;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int l = blockIdx.y;
+    const long b = blockIdx.z;
     const int m = oQ + l;
     const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
     uint64_t a1, b1, a2, b2;
@@ -313,9 +311,9 @@ int kb_bconv(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stri
 
 int kb_tensor_qr(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, const uint64_t *r1,
                  const uint64_t *r2, uint64_t *out, int batch, hipStream_t st) {
-    const long total = (long)batch * (c->qn + c->rn) * c->n;
-    hipLaunchKernelGGL(k_tensor_qr, dim3(blocks_for(total, 256)), dim3(256), 0, st, c->t, ct1, ct2, r1, r2,
-                       out, c->qn, c->rn, c->oQ, c->n, total);
+    KB_REQUIRE(batch <= 65535, "batch too large for one tensor launch");
+    hipLaunchKernelGGL(k_tensor_qr, dim3(c->n / 256, c->qn + c->rn, batch), dim3(256), 0, st, c->t, ct1, ct2,
+                       r1, r2, out, c->qn, c->rn, c->oQ, c->n);
     KB_LAUNCH_CHECK();
     return KBFV_OK;
 }
