@@ -29,7 +29,7 @@ namespace {
 
 constexpr int RG_MT = 4;                 // 16-position tiles per wave
 constexpr int RG_POS = 16 * RG_MT;       // positions per wave (one per lane in the estimate)
-constexpr int RG_WAVES = 4;              // waves per workgroup
+constexpr int RG_WAVES = 1;              // waves per workgroup
 constexpr int RG_BLOCK = RG_POS * RG_WAVES;
 constexpr int RG_KMAX = 32;              // K rows of the largest matrix
 static_assert(RG_KMAX * RG_POS * 2 >= 4 * 4 * RG_MT * 64, "W words must fit the wave's input tile");

@@ -59,11 +59,13 @@ def test_burst_is_batched_and_serial_loop_agrees(rings):
 
 
 def test_external_product_by_x_shifts_the_message(rings):
+    import torch
     import kunpbfv_amd as kb
     from kunpbfv_amd import serving
     ring = rings[13]
     r = serving.Request(0, 13, "extprod", 0.0, 2, 0)
-    ring.run("extprod", [r], keep=True)
+    # on the current stream: the decryption below reads the output there
+    ring.run("extprod", [r], stream=torch.cuda.current_stream(), keep=True)
     dec = ring.ctx.decrypt(ring.sk, r.out[None].contiguous())
     got = kb.to_host(dec)[0].astype(np.int64)
     m = ring.msgs[2].astype(np.int64)
