@@ -8,28 +8,28 @@
 // kbfv_mfma.h and one Barrett reduction per output, so the result equals the
 // scalar kernel's bit for bit.
 //
-// One wave per coefficient, no LDS and no barriers: a workgroup's 16 waves take
-// 16 consecutive coefficients of one T limb, so each 8-byte gather of a wave
-// shares its 128-byte line with the other 15 waves (the line is read from HBM
-// once).  A wave keeps its coefficient's key rows as byte-reversed digits
-// (kbfv_mfma.h, shifted per weight pass) and streams the ciphertexts through
-// in tiles of 16: 15 weight passes of MFMAs into biased accumulators (every
-// P_w + 2^22 lies in [0, 2^23]), paired into 32-bit words on the fly, then per
-// output the 128-bit sum rebuilt from its 8 words, the bias removed, and one
-// Barrett reduction.
+// One wave per coefficient at no more than 128 VGPRs: a workgroup's 8 waves
+// take 8 consecutive coefficients of one T limb, two workgroups share each CU,
+// and the two workgroups that cover one 128-byte row of 16 coefficients are
+// dispatched to the same XCD.  A wave keeps its coefficient's key rows as
+// byte-reversed digits (kbfv_mfma.h, shifted per weight pass) and streams the
+// ciphertexts through in tiles of 16: 15 weight passes of MFMAs into biased
+// accumulators (every P_w + 2^22 lies in [0, 2^23]), paired into 32-bit words
+// on the fly, then per output the 128-bit sum rebuilt from its 8 words, the
+// bias removed, and one Barrett reduction.
 #include "kbfv_internal.h"
 #include "kbfv_mfma.h"
 #include "kbfv_ops.h"
 
 namespace {
 
-constexpr int IM_WAVES = 16;              // coefficients per workgroup (one per wave)
-constexpr int IM_THREADS = 64 * IM_WAVES;  // 1024
+constexpr int IM_WAVES = 8;               // coefficients per workgroup (one per wave)
+constexpr int IM_THREADS = 64 * IM_WAVES;  // 512
 constexpr int IM_CT = 16;                 // ciphertexts per tile (one MFMA column tile)
-// output staging [r][ct][pos] in LDS: pos stride 1, ct stride 17 and row
-// stride 276 (= 4 mod 8) make both the per-wave writes and the row-wise reads
-// bank-conflict free (8-byte banks, 32 lanes per cycle)
-constexpr int IM_OC = 17, IM_OR = 16 * IM_OC + 4, IM_OBUF = 32 * IM_OR;
+// output staging [r][ct][pos] in LDS: pos stride 1, an odd ct stride and a
+// row stride of 4 mod 8 keep the per-wave writes and the row-wise reads
+// (nearly) free of bank conflicts
+constexpr int IM_OC = IM_WAVES + 1, IM_OR = 16 * IM_OC + 4, IM_OBUF = 32 * IM_OR;
 
 // Raw buffer access: a resource (base in scalar registers) plus a 32-bit byte
 // offset per lane, so an address costs one VGPR.  Every offset used here stays
@@ -68,10 +68,14 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
     const int lane = threadIdx.x & 63;
     const int p16 = lane & 15, g = lane >> 4;
     const int groups = n / IM_WAVES;
-    const int kk = blockIdx.x / groups;
+    // XCD-aware order: workgroups b and b + 8 (dispatched to the same XCD) take
+    // the two halves of one 128-byte row of 16 coefficients, so each line is
+    // fetched into one L2 only
+    const int gidx = (blockIdx.x / 16) * 16 + (blockIdx.x % 8) * 2 + (blockIdx.x / 8) % 2;
+    const int kk = gidx / groups;
     // this wave's coefficient (wave-uniform, so the base pointers below live in
     // scalar registers and every lane address is a 32-bit offset)
-    const int pos = (blockIdx.x % groups) * IM_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int pos = (gidx % groups) * IM_WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long N = n, TN = (long)tn * n;
     const uint32_t tn32 = (uint32_t)TN;  // 8 TN * 16 * 32 fits 31 bits (checked by the launcher)
     const int R2 = 2 * B;
@@ -98,9 +102,9 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
     }
 
     // ciphertext digits, cin layout [batch][D][tn][N], staged per tile of 16
-    // ciphertexts through LDS as [ct][d][pos ^ ct]: 128-byte rows of 16
-    // coefficients come in, and the 16 lanes of a wave that read one pos (its
-    // coefficient) for 16 ciphertexts hit 16 different banks
+    // ciphertexts through LDS as [ct][d][pos ^ (ct mod 8)]: 64-byte rows of 8
+    // coefficients come in, and the lanes of a wave that read one pos (its
+    // coefficient) for 16 ciphertexts spread over the banks
     const int ntiles = (batch + IM_CT - 1) / IM_CT;
     __shared__ uint64_t ibuf[IM_CT * 32 * IM_WAVES];
     __shared__ uint64_t obuf[IM_OBUF];
@@ -109,15 +113,16 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
         {
             int t = threadIdx.x;
             asm volatile("" : "+v"(t));  // recomputed per tile rather than held
-            const int pp = t & 15;
+            const int pp = t % IM_WAVES;
             const __amdgpu_buffer_rsrc_t xb = kb_rsrc(cin + (long)tile * IM_CT * D * TN + kk * N + (pos - wave));
             // rows j = 32 ct + d; rows d >= D stay unwritten (they meet zero key
             // digits) and ciphertexts past the batch repeat the last one
-            for (int j = t >> 4; j < IM_CT * 32; j += IM_THREADS / 16) {
+            for (int j = t / IM_WAVES; j < IM_CT * 32; j += IM_THREADS / IM_WAVES) {
                 const int ctl = j >> 5, d = j & 31;
                 if (d < D) {
                     const int cl = min(ctl, batch - 1 - tile * IM_CT);
-                    ibuf[j * 16 + (pp ^ ctl)] = kb_bload(xb, ((uint32_t)(cl * D + d) * tn32 + pp) * 8u);
+                    ibuf[j * IM_WAVES + (pp ^ (ctl % IM_WAVES))] =
+                        kb_bload(xb, ((uint32_t)(cl * D + d) * tn32 + pp) * 8u);
                 }
             }
         }
@@ -129,8 +134,8 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
             const int p16 = ln & 15, g = ln >> 4;
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
-                const uint64_t *src = ibuf + (p16 * 32 + 8 * c + 2 * g) * 16 + (wave ^ p16);
-                bop[c] = kb_pack2(kb_digits(src[0]), kb_digits(src[16]));
+                const uint64_t *src = ibuf + (p16 * 32 + 8 * c + 2 * g) * IM_WAVES + (wave ^ (p16 % IM_WAVES));
+                bop[c] = kb_pack2(kb_digits(src[0]), kb_digits(src[IM_WAVES]));
             }
         }
         uint64_t *const ob = obuf;
@@ -183,17 +188,17 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        // the tile's outputs leave in 128-byte rows (16 coefficients).  Both
+        // the tile's outputs leave in 64-byte rows (8 coefficients).  Both
         // buffers are reused by the next tile only after its first barrier, by
         // which every thread has finished reading them here and above
         __syncthreads();
         {
             int t = threadIdx.x;
             asm volatile("" : "+v"(t));  // recomputed per tile rather than held
-            const int pp = t & 15, ctl = (t >> 4) & 15;
+            const int pp = t % IM_WAVES, ctl = (t / IM_WAVES) % IM_CT;
             const int ctg = tile * IM_CT + ctl;
             const __amdgpu_buffer_rsrc_t og = kb_rsrc(out + (long)tile * IM_CT * R2 * TN + kk * N + (pos - wave));
-            for (int r = t >> 8; r < R2; r += IM_THREADS / 256)
+            for (int r = t / (IM_WAVES * IM_CT); r < R2; r += IM_THREADS / (IM_WAVES * IM_CT))
                 if (ctg < batch)
                     kb_bstore(og, (uint32_t)(ctl * R2 + r) * tn32 * 8u + pp * 8u, ob[r * IM_OR + ctl * IM_OC + pp]);
         }
@@ -205,7 +210,7 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
 int kb_inner_product_mfma(const kbfv_context *c, const uint64_t *cin, const uint64_t *key, uint64_t *out,
                           int batch, hipStream_t st) {
     const int D = c->dnum, B = c->ri_blocks;
-    if (D > 32 || 2 * B > 32 || c->n % IM_WAVES) return 1;
+    if (D > 32 || 2 * B > 32 || (c->tn * c->n / IM_WAVES) % 16) return 1;
     if ((long)c->tn * c->n * 8 * IM_CT * 32 >= (1L << 31)) return 1;  // 32-bit byte offsets
     const int kc = (D + 7) / 8, nj = (2 * B + 15) / 16;
     const dim3 grid((unsigned)(c->tn * (c->n / IM_WAVES)));
