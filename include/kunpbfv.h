@@ -273,6 +273,37 @@ int kbfv_external_product(const kbfv_context *ctx, const uint64_t *rgsw, const u
                           uint64_t *out, int batch, int base_bits, void *workspace,
                           size_t ws_bytes, void *stream);
 
+/* ---------------------------------------------------------------- wire format */
+
+/* Serialization of ciphertexts, plaintexts and keys.  No counterpart in the
+ * reference, which has no serialization; the format (64-byte header, then the
+ * object's words as in device memory) is specified in csrc/wire.cpp.
+ * aux: the Galois element of a Galois key, the gadget base bits of an RGSW
+ * ciphertext, otherwise 0.  Shapes per object: ciphertext and public key
+ * [2][qn][N]; plaintext [N] (mod t); secret key [pn+qn][N]; relinearisation and
+ * Galois keys [2][dnum][ri_blocks][tn][N]; RGSW [rows][2][qn][N]. */
+#define KBFV_WIRE_CIPHERTEXT 1
+#define KBFV_WIRE_PLAINTEXT 2
+#define KBFV_WIRE_SECRET_KEY 3
+#define KBFV_WIRE_PUBLIC_KEY 4
+#define KBFV_WIRE_RELIN_KEY 5
+#define KBFV_WIRE_GALOIS_KEY 6
+#define KBFV_WIRE_RGSW 7
+/* bytes of a blob holding `count` objects of `kind` (0 on bad arguments) */
+size_t kbfv_wire_bytes(const kbfv_context *ctx, int kind, int count, uint64_t aux);
+/* device objects -> blob in host memory; synchronizes the stream */
+int kbfv_wire_save(const kbfv_context *ctx, int kind, const uint64_t *dev, int count, uint64_t aux,
+                   void *buf, size_t cap, void *stream);
+/* blob -> device objects after checking magic, version, kind, count, shape,
+ * parameter fingerprint, checksum and every residue's range; synchronizes the
+ * stream (the blob may be freed on return) */
+int kbfv_wire_load(const kbfv_context *ctx, const void *buf, size_t len, int kind, uint64_t *dev,
+                   int count, void *stream);
+/* header fields of a blob without a context (kind, count, aux may be null) */
+int kbfv_wire_peek(const void *buf, size_t len, int *kind, int *count, uint64_t *aux);
+/* the payload checksum of the format */
+uint64_t kbfv_wire_checksum(const uint64_t *words, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,10 @@ class Ciphertext {
     ~Ciphertext();
 };
 
+class SecretKey;
+class Key;
+class Key_decomp;
+
 class BFVContext {
    public:
     // default parameter set of the ring degree (logN 12..16)
@@ -79,6 +83,23 @@ class BFVContext {
                       int poly_mod_len);
     void FromNTTInplace(uint64_tt *device_a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num,
                         int poly_mod_len);
+
+    // Wire format (no counterpart in the reference, which has no serialization;
+    // kunpbfv.h KBFV_WIRE_*): blobs that load only into a context with the same
+    // parameters.  A Plaintext travels as its limb-0 coefficients mod t and is
+    // replicated into every limb on load (the encode() layout).  A Key_decomp is
+    // a relinearisation key, or a Galois key when galois_elt != 0.
+    std::vector<uint8_tt> save(const Ciphertext &c);
+    std::vector<uint8_tt> save(const Plaintext &m);
+    std::vector<uint8_tt> save(const SecretKey &sk);
+    std::vector<uint8_tt> save(const Key &pk);
+    std::vector<uint8_tt> save(const Key_decomp &key, uint64_tt galois_elt = 0);
+    void load(const std::vector<uint8_tt> &blob, Ciphertext &c);
+    void load(const std::vector<uint8_tt> &blob, Plaintext &m);
+    void load(const std::vector<uint8_tt> &blob, SecretKey &sk);
+    void load(const std::vector<uint8_tt> &blob, Key &pk);
+    // returns the Galois element the blob carries (0 for a relinearisation key)
+    uint64_tt load(const std::vector<uint8_tt> &blob, Key_decomp &key);
 
     int logN, logNh, logslots, slots, L, q_num, K, dnum, alpha, p_num, t_num, r_num, mod_num, gamma;
     int Ri_blockNum, Qj_blockNum;

@@ -83,7 +83,16 @@ SIGNATURES = {
     "kbfv_rgsw_encrypt": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _sz, _vp]),
     "kbfv_external_product_workspace_bytes": (_sz, [_vp, C.c_int, C.c_int]),
     "kbfv_external_product": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _sz, _vp]),
+    "kbfv_wire_bytes": (_sz, [_vp, C.c_int, C.c_int, _u64]),
+    "kbfv_wire_save": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _u64, _vp, _sz, _vp]),
+    "kbfv_wire_load": (C.c_int, [_vp, _vp, _sz, C.c_int, _vp, C.c_int, _vp]),
+    "kbfv_wire_peek": (C.c_int, [_vp, _sz, _ip, _ip, _u64p]),
+    "kbfv_wire_checksum": (_u64, [_vp, _sz]),
 }
+
+# wire-format kinds (include/kunpbfv.h KBFV_WIRE_*)
+WIRE_CIPHERTEXT, WIRE_PLAINTEXT, WIRE_SECRET_KEY, WIRE_PUBLIC_KEY = 1, 2, 3, 4
+WIRE_RELIN_KEY, WIRE_GALOIS_KEY, WIRE_RGSW = 5, 6, 7
 
 
 class KbfvError(RuntimeError):
@@ -161,6 +170,19 @@ def to_device(a, device="cuda"):
     torch = _torch()
     a = np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
     return torch.from_numpy(a.view(np.int64)).to(device)
+
+
+def wire_peek(blob):
+    """(kind, count, aux) from a wire blob's header; no context needed"""
+    kind, count, aux = C.c_int(), C.c_int(), C.c_uint64()
+    check(lib.kbfv_wire_peek(blob, len(blob), C.byref(kind), C.byref(count), C.byref(aux)))
+    return kind.value, count.value, aux.value
+
+
+def wire_checksum(words):
+    """the format's payload checksum of a uint64 numpy array"""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    return int(lib.kbfv_wire_checksum(w.ctypes.data, w.size))
 
 
 def to_host(t):
@@ -428,6 +450,21 @@ class Context:
         r = lib.kbfv_rgsw_rows(self.h, base_bits)
         check(0 if r > 0 else r)
         return r
+
+    def wire_save(self, kind, objs, aux=0, stream=None):
+        """device objects (leading batch dimension) -> bytes of one wire blob"""
+        count = objs.shape[0]
+        nb = lib.kbfv_wire_bytes(self.h, kind, count, aux)
+        if nb == 0:
+            raise KbfvError(lib.kbfv_last_error().decode())
+        buf = C.create_string_buffer(nb)
+        check(lib.kbfv_wire_save(self.h, kind, ptr(objs), count, aux, buf, nb, stream_ptr(stream)))
+        return buf.raw
+
+    def wire_load(self, blob, kind, out, stream=None):
+        """wire blob -> the device tensor `out` (leading batch dimension)"""
+        check(lib.kbfv_wire_load(self.h, blob, len(blob), kind, ptr(out), out.shape[0], stream_ptr(stream)))
+        return out
 
     def rgsw_encrypt(self, seed, sk, m_coeff, base_bits, stream=None):
         rows = self.rgsw_rows(base_bits)

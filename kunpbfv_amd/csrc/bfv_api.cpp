@@ -465,3 +465,63 @@ void BFVScheme::multConstAndEqual(Ciphertext &cipher, Plaintext &cnst) {
     (void)n;
     hip_ok(hipStreamSynchronize(context.stream), "multConst");
 }
+
+// ---------------------------------------------------------------- wire format
+namespace {
+
+std::vector<uint8_tt> wire_save(BFVContext &c, int kind, const uint64_tt *dev, int count, uint64_t aux) {
+    const size_t nb = kbfv_wire_bytes(c.handle, kind, count, aux);
+    if (nb == 0) throw std::runtime_error(std::string("wire size: ") + kbfv_last_error());
+    std::vector<uint8_tt> blob(nb);
+    ok(kbfv_wire_save(c.handle, kind, (const uint64_t *)dev, count, aux, blob.data(), nb, c.stream), "wire save");
+    return blob;
+}
+
+void wire_load(BFVContext &c, const std::vector<uint8_tt> &blob, int kind, uint64_tt *dev, int count) {
+    ok(kbfv_wire_load(c.handle, blob.data(), blob.size(), kind, (uint64_t *)dev, count, c.stream), "wire load");
+}
+
+}  // namespace
+
+std::vector<uint8_tt> BFVContext::save(const Ciphertext &c) {
+    return wire_save(*this, KBFV_WIRE_CIPHERTEXT, c.cipher_device, 1, 0);
+}
+std::vector<uint8_tt> BFVContext::save(const Plaintext &m) {
+    return wire_save(*this, KBFV_WIRE_PLAINTEXT, m.mx_device, 1, 0);
+}
+std::vector<uint8_tt> BFVContext::save(const SecretKey &sk) {
+    return wire_save(*this, KBFV_WIRE_SECRET_KEY, sk.sx_device, 1, 0);
+}
+std::vector<uint8_tt> BFVContext::save(const Key &pk) {
+    return wire_save(*this, KBFV_WIRE_PUBLIC_KEY, pk.cipher_device, 1, 0);
+}
+std::vector<uint8_tt> BFVContext::save(const Key_decomp &key, uint64_tt galois_elt) {
+    return wire_save(*this, galois_elt ? KBFV_WIRE_GALOIS_KEY : KBFV_WIRE_RELIN_KEY, key.cipher_device, 1,
+                     galois_elt);
+}
+void BFVContext::load(const std::vector<uint8_tt> &blob, Ciphertext &c) {
+    wire_load(*this, blob, KBFV_WIRE_CIPHERTEXT, c.cipher_device, 1);
+}
+void BFVContext::load(const std::vector<uint8_tt> &blob, Plaintext &m) {
+    wire_load(*this, blob, KBFV_WIRE_PLAINTEXT, m.mx_device, 1);
+    for (int i = 1; i <= m.L; ++i)  // replicate into every limb (encode's layout)
+        hip_ok(hipMemcpyAsync(m.mx_device + (size_t)i * m.N, m.mx_device, (size_t)m.N * 8, hipMemcpyDeviceToDevice,
+                              stream),
+               "plaintext limbs");
+    hip_ok(hipStreamSynchronize(stream), "plaintext limbs");
+}
+void BFVContext::load(const std::vector<uint8_tt> &blob, SecretKey &sk) {
+    wire_load(*this, blob, KBFV_WIRE_SECRET_KEY, sk.sx_device, 1);
+}
+void BFVContext::load(const std::vector<uint8_tt> &blob, Key &pk) {
+    wire_load(*this, blob, KBFV_WIRE_PUBLIC_KEY, pk.cipher_device, 1);
+}
+uint64_tt BFVContext::load(const std::vector<uint8_tt> &blob, Key_decomp &key) {
+    int kind = 0, count = 0;
+    uint64_t aux = 0;
+    ok(kbfv_wire_peek(blob.data(), blob.size(), &kind, &count, &aux), "wire peek");
+    if (kind != KBFV_WIRE_RELIN_KEY && kind != KBFV_WIRE_GALOIS_KEY)
+        throw std::invalid_argument("wire blob holds no key-switching key");
+    wire_load(*this, blob, kind, key.cipher_device, 1);
+    return kind == KBFV_WIRE_GALOIS_KEY ? aux : 0;
+}

@@ -113,6 +113,38 @@ int main(int argc, char *argv[]) {
     for (int i = 0; i < slots; ++i) w[i] = (uint64_tt)((unsigned __int128)m1[i] * m1[i] % t);
     bad += check("squareAndEqual", c, w);
 
+    // wire format: ciphertext, plaintext and secret key through blobs
+    fresh();
+    {
+        std::vector<uint8_tt> blob = context.save(c);
+        Ciphertext back(N, L, L, slots);
+        context.load(blob, back);
+        bad += check("wire ciphertext", back, m1);
+        SecretKey sk2(context);
+        context.load(context.save(sk), sk2);
+        scheme.decryptMsg(pd, sk2, back);
+        context.decode(pd, dd);
+        std::vector<uint64_tt> got(slots);
+        (void)hipMemcpy(got.data(), dd, 8 * slots, hipMemcpyDeviceToHost);
+        int wrong = 0;
+        for (int i = 0; i < slots; ++i) wrong += got[i] != m1[i];
+        printf("%-22s slots_wrong=%d\n", "wire secret key", wrong);
+        bad += wrong;
+        Plaintext pl(N, L, L, slots);
+        context.load(context.save(p2), pl);
+        scheme.encryptMsg(c2, pl);
+        bad += check("wire plaintext", c2, m2);
+        int rejected = 0;
+        try {
+            Key pk_wrong(N, L);
+            context.load(blob, pk_wrong);  // a ciphertext blob is no public key
+        } catch (const std::runtime_error &) {
+            rejected = 1;
+        }
+        printf("%-22s %s\n", "wire kind check", rejected ? "rejected" : "ACCEPTED");
+        bad += !rejected;
+    }
+
     printf("N=%d total_wrong=%d\n", N, bad);
     (void)hipFree(d1);
     (void)hipFree(d2);

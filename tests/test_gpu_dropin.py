@@ -29,7 +29,8 @@ OPS = os.path.join(ROOT, "kunpbfv_amd", "examples", "drop_in_ops")
 @pytest.mark.parametrize("n", [4096, 16384])
 def test_drop_in_ops_driver(n):
     """rotation, conjugation, negation, constant multiply/add, plaintext add and
-    square through the reference's BFVScheme member names"""
+    square through the reference's BFVScheme member names, and ciphertext,
+    plaintext and secret key through wire blobs"""
     assert os.path.exists(OPS), "example driver not built (run __graft_entry__.build())"
     r = subprocess.run([OPS, str(n)], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
