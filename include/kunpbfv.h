@@ -196,6 +196,22 @@ int kbfv_mult_relin(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t
                     const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
                     size_t ws_bytes, void *stream);
 
+/* The reference's split-IP seam, batched (BFVScheme::multAndEqual_beforeIP_23 /
+ * multAndEqual_afterIP_23, include/KeySwitch_23.cuh:312-372).  before_ip runs
+ * the tensor half of HomMult and writes the inner product's input ip_in
+ * [batch][dnum][tn][N] (digits of d2 over T, NTT form) and the linear terms
+ * d01 [batch][2][qn][N] (d1 = a1b2 + a2b1, d0 = b1b2, coefficient form);
+ * kbfv_inner_product (key-stationary over the batch, the reference's
+ * external_product_T_swk_reuse, include/ExternalProduct.cuh:118-177) maps
+ * ip_in to ip_out [batch][2][ri_blocks][tn][N]; after_ip finishes the
+ * relinearised product into out [batch][2][qn][N] (ip_out is not modified).
+ * The three calls give kbfv_mult_relin's result bit for bit.  Workspace:
+ * kbfv_mult_workspace_bytes. */
+int kbfv_mult_before_ip(const kbfv_context *ctx, const uint64_t *ct1, const uint64_t *ct2, uint64_t *ip_in,
+                        uint64_t *d01, int batch, void *workspace, size_t ws_bytes, void *stream);
+int kbfv_mult_after_ip(const kbfv_context *ctx, const uint64_t *ip_out, const uint64_t *d01, uint64_t *out,
+                       int batch, void *workspace, size_t ws_bytes, void *stream);
+
 /* HIP-graph form of kbfv_mult_relin: the batched pipeline for fixed buffers is
  * captured once and each later kbfv_graph_launch replays every kernel with a
  * single submission.  No counterpart in the reference, which launches every

@@ -84,6 +84,14 @@ class BFVContext {
     void FromNTTInplace(uint64_tt *device_a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num,
                         int poly_mod_len);
 
+    // key-stationary inner product over a batch (include/ExternalProduct.cuh:169):
+    // cipher_modUp_QjtoT [batch_size][Qj_blockNum][t_num][N] (NTT over T, as
+    // BFVScheme::multAndEqual_beforeIP_23 writes it) times swk_modUp_RitoT (a
+    // Key_decomp's cipher_device) -> output [batch_size][2][Ri_blockNum][t_num][N];
+    // BFV ciphertexts stay at full level, so l must be L
+    void external_product_T_swk_reuse(uint64_tt *output, uint64_tt *cipher_modUp_QjtoT, uint64_tt *swk_modUp_RitoT,
+                                      int l, int batch_size);
+
     // Wire format (no counterpart in the reference, which has no serialization;
     // kunpbfv.h KBFV_WIRE_*): blobs that load only into a context with the same
     // parameters.  A Plaintext travels as its limb-0 coefficients mod t and is
@@ -181,6 +189,16 @@ class BFVScheme {
     void multAndEqual_23(Ciphertext &cipher1, Ciphertext &cipher2);
     // the batched form: every kernel covers all pairs, the key is read once
     void multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector<Ciphertext *> &c2);
+    // the split-IP seam (include/KeySwitch_23.cuh:312-372): beforeIP writes the
+    // pair's inner-product input (Qj_blockNum*t_num*N words) and its linear terms
+    // (q_num*N words each, opaque to the caller); once
+    // context.external_product_T_swk_reuse has run over a batch of such inputs,
+    // afterIP finishes cipher1 = cipher1 * cipher2 relinearised, adding the
+    // linear terms back (the reference leaves that addition commented out)
+    void multAndEqual_beforeIP_23(Ciphertext &cipher1, Ciphertext &cipher2, uint64_tt *IP_input,
+                                  uint64_tt *axbx1_mul_batch, uint64_tt *bxbx_mul_batch);
+    void multAndEqual_afterIP_23(Ciphertext &cipher1, Ciphertext &cipher2, uint64_tt *IP_output,
+                                 uint64_tt *axbx1_mul_batch, uint64_tt *bxbx_mul_batch);
     // plaintext (from encode_ntt) times ciphertext (include/BFVScheme.cuh:226)
     void multConstAndEqual(Ciphertext &cipher, Plaintext &cnst);
     void multConstAndEqual(Ciphertext &cipher, uint64_tt cnst);
@@ -208,4 +226,5 @@ class BFVScheme {
     uint64_t seed_state = 0x6b756e7042465600ULL;
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    uint64_tt *d01 = nullptr;  // [2][q_num][N] linear terms of the split-IP seam
 };

@@ -83,6 +83,8 @@ SIGNATURES = {
     "kbfv_rgsw_encrypt": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _vp, _sz, _vp]),
     "kbfv_external_product_workspace_bytes": (_sz, [_vp, C.c_int, C.c_int]),
     "kbfv_external_product": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _sz, _vp]),
+    "kbfv_mult_before_ip": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_mult_after_ip": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
     "kbfv_wire_bytes": (_sz, [_vp, C.c_int, C.c_int, _u64]),
     "kbfv_wire_save": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _u64, _vp, _sz, _vp]),
     "kbfv_wire_load": (C.c_int, [_vp, _vp, _sz, C.c_int, _vp, C.c_int, _vp]),
@@ -380,6 +382,32 @@ class Context:
         ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
         check(lib.kbfv_mult_relin(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
                                   stream_ptr(stream)))
+        return out
+
+    def mult_before_ip(self, c1, c2, ws=None, stream=None):
+        """split-IP seam, first half: (ip_in [batch][dnum][tn][N], d01 [batch][2][qn][N])"""
+        batch = c1.shape[0]
+        ip_in = self.empty(batch, self.dnum, self.tn, self.n)
+        d01 = self.empty(batch, 2, self.qn, self.n)
+        ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
+        check(lib.kbfv_mult_before_ip(self.h, ptr(c1), ptr(c2), ptr(ip_in), ptr(d01), batch, ptr(ws), ws.numel(),
+                                      stream_ptr(stream)))
+        return ip_in, d01
+
+    def ip_batch(self, ip_in, key, stream=None):
+        """key-stationary inner product over the batch -> [batch][2][ri_blocks][tn][N]"""
+        batch = ip_in.shape[0]
+        out = self.empty(batch, 2, self.ri_blocks, self.tn, self.n)
+        check(lib.kbfv_inner_product(self.h, ptr(ip_in), ptr(key), ptr(out), batch, stream_ptr(stream)))
+        return out
+
+    def mult_after_ip(self, ip_out, d01, out=None, ws=None, stream=None):
+        """split-IP seam, second half: the relinearised products [batch][2][qn][N]"""
+        batch = ip_out.shape[0]
+        out = out if out is not None else self.empty(batch, 2, self.qn, self.n)
+        ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
+        check(lib.kbfv_mult_after_ip(self.h, ptr(ip_out), ptr(d01), ptr(out), batch, ptr(ws), ws.numel(),
+                                     stream_ptr(stream)))
         return out
 
     def negate(self, ct, out=None, stream=None):

@@ -169,6 +169,16 @@ void BFVContext::FromNTTInplace(uint64_tt *a, int start_poly_idx, int start_mod_
        "FromNTTInplace");
 }
 
+void BFVContext::external_product_T_swk_reuse(uint64_tt *output, uint64_tt *cipher_modUp_QjtoT,
+                                              uint64_tt *swk_modUp_RitoT, int l, int batch_size) {
+    if (l != L) throw std::invalid_argument("external_product_T_swk_reuse: ciphertexts are at full level");
+    if (batch_size <= 0) return;
+    ok(kbfv_inner_product(handle, (const uint64_t *)cipher_modUp_QjtoT, (const uint64_t *)swk_modUp_RitoT,
+                          (uint64_t *)output, batch_size, stream),
+       "external_product_T_swk_reuse");
+    hip_ok(hipStreamSynchronize(stream), "external_product_T_swk_reuse");
+}
+
 // ---------------------------------------------------------------- keys
 static uint64_t splitmix(uint64_t &s) {
     uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
@@ -220,6 +230,7 @@ BFVScheme::~BFVScheme() {
     for (auto &kv : rotKey_vec_23) delete kv.second;
     delete ConjKey_23;
     dfree(ws);
+    dfree(d01);
 }
 
 void BFVScheme::setSeed(uint64_t seed) { seed_state = seed; }
@@ -408,6 +419,40 @@ void BFVScheme::mult_23(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
 }
 
 void BFVScheme::multAndEqual_23(Ciphertext &c1, Ciphertext &c2) { mult_23(c1, c1, c2); }
+
+// The linear terms travel between the halves in the caller's two buffers (the
+// reference's layout); the C ABI keeps them as one [2][q_num][N] block, so
+// they are copied through the scheme-owned d01 on either side.
+void BFVScheme::multAndEqual_beforeIP_23(Ciphertext &c1, Ciphertext &c2, uint64_tt *IP_input,
+                                         uint64_tt *axbx1_mul_batch, uint64_tt *bxbx_mul_batch) {
+    if (c1.l == 0 || c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    if (!rlk_23) throw std::invalid_argument("rotKey_23 not exists");
+    c1.l = std::min(c1.l, c2.l);
+    const size_t poly = (size_t)context.N * context.q_num;
+    if (!d01) d01 = dalloc(2 * poly);
+    void *w = ensure_ws(kbfv_mult_workspace_bytes(context.handle, 1));
+    ok(kbfv_mult_before_ip(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
+                           (uint64_t *)IP_input, (uint64_t *)d01, 1, w, ws_bytes, context.stream),
+       "multAndEqual_beforeIP_23");
+    hip_ok(hipMemcpyAsync(axbx1_mul_batch, d01, poly * 8, hipMemcpyDeviceToDevice, context.stream), "beforeIP");
+    hip_ok(hipMemcpyAsync(bxbx_mul_batch, d01 + poly, poly * 8, hipMemcpyDeviceToDevice, context.stream), "beforeIP");
+    hip_ok(hipStreamSynchronize(context.stream), "multAndEqual_beforeIP_23");
+}
+
+void BFVScheme::multAndEqual_afterIP_23(Ciphertext &c1, Ciphertext &c2, uint64_tt *IP_output,
+                                        uint64_tt *axbx1_mul_batch, uint64_tt *bxbx_mul_batch) {
+    if (c1.l == 0 || c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    c1.l = std::min(c1.l, c2.l);
+    const size_t poly = (size_t)context.N * context.q_num;
+    if (!d01) d01 = dalloc(2 * poly);
+    hip_ok(hipMemcpyAsync(d01, axbx1_mul_batch, poly * 8, hipMemcpyDeviceToDevice, context.stream), "afterIP");
+    hip_ok(hipMemcpyAsync(d01 + poly, bxbx_mul_batch, poly * 8, hipMemcpyDeviceToDevice, context.stream), "afterIP");
+    void *w = ensure_ws(kbfv_mult_workspace_bytes(context.handle, 1));
+    ok(kbfv_mult_after_ip(context.handle, (const uint64_t *)IP_output, (const uint64_t *)d01,
+                          (uint64_t *)c1.cipher_device, 1, w, ws_bytes, context.stream),
+       "multAndEqual_afterIP_23");
+    hip_ok(hipStreamSynchronize(context.stream), "multAndEqual_afterIP_23");
+}
 
 void BFVScheme::multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector<Ciphertext *> &c2) {
     if (c1.size() != c2.size()) throw std::invalid_argument("batch sizes differ");

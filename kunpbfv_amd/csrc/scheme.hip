@@ -523,6 +523,65 @@ extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const
     return KBFV_OK;
 }
 
+// ---------------------------------------------------------------- split-IP seam
+// The reference's multAndEqual_beforeIP_23 / multAndEqual_afterIP_23
+// (include/KeySwitch_23.cuh:312-372), batched, around kbfv_inner_product.
+extern "C" int kbfv_mult_before_ip(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2,
+                                   uint64_t *ip_in, uint64_t *d01, int batch, void *ws, size_t ws_bytes,
+                                   void *st) {
+    KB_REQUIRE(c && ct1 && ct2 && ip_in && d01 && ws && batch > 0, "null argument");
+    int cb = batch;
+    while (cb > 1 && mult_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
+    KB_REQUIRE(mult_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
+    MultWs w;
+    mult_layout(c, cb, ws, &w);
+    const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn, CTS = 2 * qn * N;
+    const long IPS = (long)c->dnum * c->tn * N;
+    hipStream_t s = S(st);
+    for (int b0 = 0; b0 < batch; b0 += cb) {
+        const int n = std::min(cb, batch - b0);
+        const uint64_t *a = ct1 + b0 * CTS, *b = ct2 + b0 * CTS;
+        // steps 1-8 of the fused pipeline (mult_chunk), the digits going to ip_in
+        KB_TRY(kb_ntt(c, -1, a, w.X1, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_ntt(c, -1, b, w.X2, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 2 * n, s));
+        KB_TRY(kb_bconv(c, true, w.X2, qn * N, w.R2, rn * N, 2 * n, s));
+        KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * n, 1, 0, rn * N, N, 0, rn * N, N, s));
+        KB_TRY(kb_tensor_qr(c, a, b, w.R1, w.R2, w.D3, n, s));
+        KB_TRY(kb_ntt(c, -1, w.D3, w.D3, c->oQ, sn, 3 * n, 1, 0, sn * N, N, 0, sn * N, N, s));
+        KB_TRY(kb_scale_round(c, w.D3, w.SR, 3 * n, s));
+        KB_TRY(kb_bconv(c, false, w.SR, rn * N, w.DQ, qn * N, 3 * n, s));
+        KB_TRY(kb_digits_to_t_ntt(c, w.DQ, 3 * qn * N, ip_in + b0 * IPS, n, nullptr, s));
+        // d1 (a1b2 + a2b1) and d0 (b1b2), coefficient form: polys 1 and 2 of each triple
+        KB_HIP(hipMemcpy2DAsync(d01 + b0 * CTS, CTS * 8, w.DQ + qn * N, 3 * qn * N * 8, CTS * 8, n,
+                                hipMemcpyDeviceToDevice, s));
+    }
+    return KBFV_OK;
+}
+
+extern "C" int kbfv_mult_after_ip(const kbfv_context *c, const uint64_t *ip_out, const uint64_t *d01,
+                                  uint64_t *out, int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && ip_out && d01 && out && ws && batch > 0, "null argument");
+    int cb = batch;
+    while (cb > 1 && mult_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
+    KB_REQUIRE(mult_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
+    MultWs w;
+    mult_layout(c, cb, ws, &w);
+    const long N = c->n, qn = c->qn, CTS = 2 * qn * N;
+    const long IPO = 2L * c->ri_blocks * c->tn * N;
+    hipStream_t s = S(st);
+    for (int b0 = 0; b0 < batch; b0 += cb) {
+        const int n = std::min(cb, batch - b0);
+        // steps 10-13 of the fused pipeline; the caller's inner-product output is left as it is
+        KB_TRY(kb_ntt(c, -1, ip_out + b0 * IPO, w.IP, c->oT, c->tn, 2 * c->ri_blocks * n, 1, 0,
+                      (long)c->tn * N, N, 0, (long)c->tn * N, N, s));
+        KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, n, s));
+        KB_TRY(kb_moddown(c, w.PQ, w.MD, d01 + b0 * CTS, CTS, n, s));
+        KB_TRY(kb_ntt(c, +1, w.MD, out + b0 * CTS, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+    }
+    return KBFV_OK;
+}
+
 // ---------------------------------------------------------------- Galois automorphisms
 // (BFVScheme::addLeftRotKey_23 / addConjKey_23 / leftRotateAndEqual_23 /
 // conjugate_23, include/KeySwitch_23.cuh:70-166, 466-584)

@@ -145,6 +145,41 @@ int main(int argc, char *argv[]) {
         bad += !rejected;
     }
 
+    // split-IP seam over two pairs with one key-stationary inner product
+    // between the halves (the reference's include/advanced/SchemeAlgo.cuh:460-490)
+    {
+        const size_t ipi = (size_t)context.Qj_blockNum * context.t_num * N;
+        const size_t ipo = (size_t)2 * context.Ri_blockNum * context.t_num * N;
+        const size_t lin = (size_t)context.q_num * N;
+        uint64_tt *ip_in, *ip_out, *axbx1, *bxbx;
+        (void)hipMalloc(&ip_in, 8 * 2 * ipi);
+        (void)hipMalloc(&ip_out, 8 * 2 * ipo);
+        (void)hipMalloc(&axbx1, 8 * 2 * lin);
+        (void)hipMalloc(&bxbx, 8 * 2 * lin);
+        Ciphertext a0(N, L, L, slots), b0(N, L, L, slots), a1(N, L, L, slots), b1(N, L, L, slots);
+        scheme.encryptMsg(a0, p1);
+        scheme.encryptMsg(b0, p2);
+        scheme.encryptMsg(a1, p2);
+        scheme.encryptMsg(b1, p2);
+        Ciphertext *x[2] = {&a0, &a1}, *y[2] = {&b0, &b1};
+        for (int i = 0; i < 2; ++i)
+            scheme.multAndEqual_beforeIP_23(*x[i], *y[i], ip_in + i * ipi, axbx1 + i * lin, bxbx + i * lin);
+        context.external_product_T_swk_reuse(ip_out, ip_in, scheme.rlk_23->cipher_device, L, 2);
+        for (int i = 0; i < 2; ++i)
+            scheme.multAndEqual_afterIP_23(*x[i], *y[i], ip_out + i * ipo, axbx1 + i * lin, bxbx + i * lin);
+        std::vector<uint64_tt> w0(slots), w1(slots);
+        for (int i = 0; i < slots; ++i) {
+            w0[i] = (uint64_tt)((unsigned __int128)m1[i] * m2[i] % t);
+            w1[i] = (uint64_tt)((unsigned __int128)m2[i] * m2[i] % t);
+        }
+        bad += check("split-IP pair 0", a0, w0);
+        bad += check("split-IP pair 1", a1, w1);
+        (void)hipFree(ip_in);
+        (void)hipFree(ip_out);
+        (void)hipFree(axbx1);
+        (void)hipFree(bxbx);
+    }
+
     printf("N=%d total_wrong=%d\n", N, bad);
     (void)hipFree(d1);
     (void)hipFree(d2);

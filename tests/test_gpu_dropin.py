@@ -29,8 +29,9 @@ OPS = os.path.join(ROOT, "kunpbfv_amd", "examples", "drop_in_ops")
 @pytest.mark.parametrize("n", [4096, 16384])
 def test_drop_in_ops_driver(n):
     """rotation, conjugation, negation, constant multiply/add, plaintext add and
-    square through the reference's BFVScheme member names, and ciphertext,
-    plaintext and secret key through wire blobs"""
+    square through the reference's BFVScheme member names, ciphertext,
+    plaintext and secret key through wire blobs, and two products through the
+    split-IP seam (beforeIP, one batched external_product_T_swk_reuse, afterIP)"""
     assert os.path.exists(OPS), "example driver not built (run __graft_entry__.build())"
     r = subprocess.run([OPS, str(n)], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
