@@ -12,8 +12,12 @@
 //                   global stages 0 .. log2(N1)-1
 //   row pass     -- C contiguous chunks of N2 elements, remaining stages
 // Forward = column then row; inverse = row then column.  Values stay lazy in
-// [0, 4q) (forward) / [0, 2q) (inverse) between stages and are reduced once on
-// the final store.  Twiddles are (w, Shoup w') pairs read as one 16-byte load,
+// [0, 8q) (forward) / [0, 4q) (inverse) between stages (moduli are below 2^60,
+// so every sum stays below 2^63) and are reduced once on the final store.  The
+// twiddle products use an under-estimated Shoup quotient (one 32-bit high
+// product and two carries fewer, result in [0, 4q)), and conditional
+// subtractions select on the sign word with v_bfi_b32, so no carry or compare
+// result passes through an SGPR.  Twiddles are (w, Shoup w') pairs read as one 16-byte load,
 // one load per distinct twiddle per thread.  The grid is ordered limb-slowest so
 // that every workgroup in flight shares one modulus' tables in L2.
 #include "kbfv_internal.h"
@@ -38,19 +42,93 @@ struct Geo {
     }
 };
 
-// values stay below 4q < 2^62; nq = 2^64 - q
-__device__ __forceinline__ void ct_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t nq, uint64_t q2) {
-    a = kb_csub_s(a, q2);                             // [0, 2q)
-    const uint64_t t = kb_shoup_lazy_nq(b, w.w, w.ws, nq);  // [0, 2q)
-    b = a + q2 - t;
-    a = a + t;
+// 64-bit value + 32-bit value as one v_mad_u64_u32 (b * 1 + a): no carry
+// through an SGPR and no register pair to build for b
+__device__ __forceinline__ uint64_t add_u32(uint64_t a, uint32_t b) {
+    uint32_t one = 1;
+    asm("" : "+s"(one));  // a multiplier the optimizer cannot fold away
+    return kb_mad32(b, one, a);
 }
 
-__device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t nq, uint64_t q2) {
-    const uint64_t u = kb_csub_s(a + b, q2);
-    const uint64_t v = kb_shoup_lazy_nq(a + q2 - b, w.w, w.ws, nq);
+// Shoup product x*w mod q with an under-estimated quotient, in [0, 4q) for any
+// x < 2^64.  floor(x*ws / 2^64) = x1*s1 + hi(x1*s0) + hi(x0*s1) + e with
+// e in {0, 1, 2} (the carries of the low words and hi(x0*s0)); leaving e out
+// adds at most 2q to the lazy residue x*w - h*q in [0, 2q).  nq = 2^64 - q.
+__device__ __forceinline__ uint64_t shoup4(uint64_t x, uint64_t w, uint64_t ws, uint64_t nq) {
+    const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32);
+    const uint32_t s0 = (uint32_t)ws, s1 = (uint32_t)(ws >> 32);
+    const uint64_t h = add_u32(kb_mad32(x1, s1, __umulhi(x1, s0)), __umulhi(x0, s1));
+    const uint32_t h0 = (uint32_t)h, h1 = (uint32_t)(h >> 32);
+    const uint32_t w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+    const uint32_t n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+    const uint64_t r = kb_mad32(h0, n0, kb_mad32(x0, w0, 0));
+    const uint32_t cross = kb_mul_lo32(x0, w1) + kb_mul_lo32(x1, w0) + kb_mul_lo32(h0, n1) +
+                           kb_mul_lo32(h1, n0);
+    return r + ((uint64_t)cross << 32);
+}
+
+// a >= m ? a - m : a for a, m < 2^63, given nm = 2^64 - m: the sign of a - m
+// selects word by word (v_bfi_b32), no compare or carry through an SGPR
+__device__ __forceinline__ uint64_t csub_sel(uint64_t a, uint64_t nm) {
+    const uint64_t t = a + nm;
+    const uint32_t msk = (uint32_t)((int32_t)(uint32_t)(t >> 32) >> 31);
+    uint32_t lo, hi;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(lo) : "v"(msk), "v"((uint32_t)a), "v"((uint32_t)t));
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(hi) : "v"(msk), "v"((uint32_t)(a >> 32)), "v"((uint32_t)(t >> 32)));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// forward values stay in [0, 8q), inverse values in [0, 4q); q < 2^60 keeps
+// every sum below 2^63.  nq = 2^64 - q, q4 = 4q, nq4 = 2^64 - 4q.
+__device__ __forceinline__ void ct_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t nq, uint64_t q4,
+                                        uint64_t nq4) {
+    a = csub_sel(a, nq4);                        // [0, 4q)
+    const uint64_t t = shoup4(b, w.w, w.ws, nq);  // [0, 4q)
+    b = a + q4 - t;                              // (0, 8q)
+    a = a + t;                                   // [0, 8q)
+}
+
+__device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uint64_t nq, uint64_t q4,
+                                        uint64_t nq4) {
+    const uint64_t u = csub_sel(a + b, nq4);               // [0, 4q)
+    const uint64_t v = shoup4(a + q4 - b, w.w, w.ws, nq);  // [0, 4q)
     a = u;
     b = v;
+}
+
+// The twiddles of the one butterfly group thread threadIdx.x takes in a round
+// (rounds with exactly one group per thread), in the order the round consumes
+// them: level j of the round, sub-block m -> slot (1 << j) - 1 + m.
+template <int LR, int LOG_S, bool COLUMN>
+__device__ __forceinline__ void round_twiddles(kbfv_tw *tws, const int lam0, const kbfv_tw *tab,
+                                               const int log_n, const int g0, const int grp0,
+                                               const int stride) {
+    using G = Geo<LOG_S, COLUMN>;
+    constexpr int S = G::S, C = G::C, RR = 1 << LR;
+    const int grp = threadIdx.x;
+    int c, u;
+    if constexpr (COLUMN) {
+        c = grp % C;
+        u = grp / C;
+    } else {
+        u = grp % (S >> LR);
+        c = grp / (S >> LR);
+    }
+    const int Bs = S >> lam0;
+    const int d = Bs >> LR;
+    const int kbase = (u / d) * Bs + (u % d);
+    const int posc = COLUMN ? (grp0 * C + c) : (grp0 * C + c) * S;
+    const int pstep = COLUMN ? stride : 1;
+#pragma unroll
+    for (int j = 0; j < LR; ++j) {
+        const int h = RR >> (j + 1);
+        const int gst = g0 + lam0 + j;
+#pragma unroll
+        for (int m = 0; m < (1 << j); ++m) {
+            const int pos = posc + (kbase + m * (2 * h) * d) * pstep;
+            tws[(1 << j) - 1 + m] = tab[(1 << gst) + (pos >> (log_n - gst))];
+        }
+    }
 }
 
 // One register round of LR levels starting at pass level lam0.  The round reads
@@ -59,16 +137,21 @@ __device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uin
 // 128-byte runs); it writes them back to the tile or, when GOUT, straight to
 // global memory with the final reduction of the last pass (FIN).
 template <int LR, int LOG_S, bool COLUMN, bool FWD, bool NINV, bool GIN = false, bool GOUT = false,
-          bool FIN = false>
+          bool FIN = false, bool PRE = false>
 __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const kbfv_tw *tab,
                                           const int log_n, const int g0, const int grp0,
                                           const int stride, const uint64_t q, const kbfv_tw ninv,
                                           const uint64_t *__restrict__ gin = nullptr,
-                                          uint64_t *__restrict__ gout = nullptr) {
+                                          uint64_t *__restrict__ gout = nullptr,
+                                          const kbfv_tw (*pre)[(1 << LR) - 1] = nullptr) {
     using G = Geo<LOG_S, COLUMN>;
     constexpr int S = G::S, C = G::C, RR = 1 << LR;
     constexpr int NG = C * (S >> LR);
-    const uint64_t q2 = q << 1, nq = 0 - q;
+    const uint64_t q4 = q << 2;
+    uint64_t nq = 0 - q, nq4 = 0 - q4;
+    // opaque to the optimizer, which would otherwise turn x + (2^64 - m) back
+    // into a subtraction with its carry through an SGPR
+    asm volatile("" : "+s"(nq), "+s"(nq4));
     const int Bs = S >> lam0;
     const int d = Bs >> LR;
     for (int grp = threadIdx.x; grp < NG; grp += G::T) {
@@ -101,9 +184,13 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
                 for (int m = 0; m < (1 << j); ++m) {
                     const int i0 = m * (2 * h);
                     const int pos = posc + (kbase + i0 * d) * pstep;
-                    const kbfv_tw w = tab[(1 << gst) + (pos >> (log_n - gst))];
+                    kbfv_tw w;
+                    if constexpr (PRE)
+                        w = (*pre)[(1 << j) - 1 + m];
+                    else
+                        w = tab[(1 << gst) + (pos >> (log_n - gst))];
 #pragma unroll
-                    for (int i = i0; i < i0 + h; ++i) ct_bfly(x[i], x[i + h], w, nq, q2);
+                    for (int i = i0; i < i0 + h; ++i) ct_bfly(x[i], x[i + h], w, nq, q4, nq4);
                 }
             }
         } else {
@@ -115,11 +202,15 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
                 for (int m = 0; m < (1 << j); ++m) {
                     const int i0 = m * (2 * h);
                     const int pos = posc + (kbase + i0 * d) * pstep;
-                    const kbfv_tw w = tab[(1 << gst) + (pos >> (log_n - gst))];
+                    kbfv_tw w;
+                    if constexpr (PRE)
+                        w = (*pre)[(1 << j) - 1 + m];
+                    else
+                        w = tab[(1 << gst) + (pos >> (log_n - gst))];
 #pragma unroll
                     for (int i = i0; i < i0 + h; ++i) {
-                        gs_bfly(x[i], x[i + h], w, nq, q2);
-                        if (NINV && gst == 0) x[i] = kb_shoup_lazy_nq(x[i], ninv.w, ninv.ws, nq);
+                        gs_bfly(x[i], x[i + h], w, nq, q4, nq4);
+                        if (NINV && gst == 0) x[i] = shoup4(x[i], ninv.w, ninv.ws, nq);
                     }
                 }
             }
@@ -129,8 +220,9 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
             if constexpr (GOUT) {
                 uint64_t v = x[i];
                 if constexpr (FIN) {
-                    if constexpr (FWD) v = kb_csub(v, q2);
-                    v = kb_csub(v, q);
+                    if constexpr (FWD) v = csub_sel(v, nq4);
+                    v = csub_sel(v, 0 - (q << 1));
+                    v = csub_sel(v, nq);
                 }
                 gout[posc + (kbase + i * d) * pstep] = v;
             } else {
@@ -161,7 +253,6 @@ __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restr
         const int pos = COLUMN ? (grp0 * C + c + k * stride) : ((grp0 * C + c) * S + k);
         tile[G::addr(c, k)] = in[pos];
     }
-    __syncthreads();
 }
 
 // pass sizes: the column pass takes ceil(log_n / 2) levels, the row pass the rest
@@ -183,7 +274,11 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     constexpr bool LAST = (FWD != COLUMN);  // forward ends with the row pass
     __shared__ uint64_t tile[G::WORDS];
 
-    int bid = blockIdx.x;
+    // XCD-aware order: workgroup b runs on XCD b % 8, so when the grid is a
+    // multiple of 8 each XCD gets a contiguous range of tiles (the column
+    // tiles that share DRAM rows, and one modulus' twiddles, stay in one L2)
+    int bid = (int)blockIdx.x;
+    if ((gridDim.x & 7) == 0) bid = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
     const int grp0 = bid % groups;
     bid /= groups;
     const int y = bid % polys;
@@ -222,8 +317,24 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
             ntt_round<LR1, LOG_S, COLUMN, false, NI, true>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv,
                                                            in);
         } else {
-            stage_in<LOG_S, COLUMN>(tile, in, grp0, stride);
-            ntt_round<LR1, LOG_S, COLUMN, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
+            // one butterfly group per thread (a radix-16 round): its twiddles are
+            // loaded with the tile, ahead of the barrier, rather than after it
+            constexpr bool PRE = (C * (S >> LR1) == T);
+            if constexpr (PRE) {
+                kbfv_tw pre[(1 << LR1) - 1];
+                round_twiddles<LR1, LOG_S, COLUMN>(pre, LR0, tab, log_n, g0, grp0, stride);
+                stage_in<LOG_S, COLUMN>(tile, in, grp0, stride);
+                // pinned here so that the loads are not sunk below the barrier
+#pragma unroll
+                for (int i = 0; i < (1 << LR1) - 1; ++i) asm volatile("" : "+v"(pre[i].w), "+v"(pre[i].ws));
+                __syncthreads();
+                ntt_round<LR1, LOG_S, COLUMN, false, NI, false, false, false, true>(
+                    tile, LR0, tab, log_n, g0, grp0, stride, q, ninv, nullptr, nullptr, &pre);
+            } else {
+                stage_in<LOG_S, COLUMN>(tile, in, grp0, stride);
+                __syncthreads();
+                ntt_round<LR1, LOG_S, COLUMN, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
+            }
         }
         ntt_round<LR0, LOG_S, COLUMN, false, NI, false, true, LAST>(tile, 0, tab, log_n, g0, grp0, stride, q,
                                                                     ninv, nullptr, out);
@@ -231,15 +342,15 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     }
 
     // forward row pass: stride-1 round done in the tile, coalesced store from it
-    const uint64_t q2 = q << 1;
 #pragma unroll
     for (int j = 0; j < TILE / T; ++j) {
         const int e = threadIdx.x + j * T;
         const int c = e / S, k = e % S;
         uint64_t v = tile[G::addr(c, k)];
         if constexpr (LAST) {
-            v = kb_csub(v, q2);
-            v = kb_csub(v, q);
+            v = csub_sel(v, 0 - (q << 2));
+            v = csub_sel(v, 0 - (q << 1));
+            v = csub_sel(v, 0 - q);
         }
         out[(grp0 * C + c) * S + k] = v;
     }
