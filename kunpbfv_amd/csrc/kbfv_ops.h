@@ -36,6 +36,12 @@ int kb_inner_product_mfma(const kbfv_context *c, const uint64_t *cin, const uint
 int kb_modup_t_to_pq(const kbfv_context *c, const uint64_t *in, uint64_t *out, int batch, hipStream_t st);
 int kb_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
                long add_stride, int batch, hipStream_t st);
+// kb_modup_t_to_pq then kb_moddown in one kernel, without the PQ intermediate,
+// for the shapes kb_modup_moddown_fused accepts; other shapes run the two
+// kernels through pq_tmp ([batch][2][pn+qn][N], unused and may be null when fused)
+bool kb_modup_moddown_fused(const kbfv_context *c);
+int kb_modup_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
+                     long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st);
 // digit decomposition of the Q limbs into T followed by the forward NTT over T
 int kb_digits_to_t_ntt(const kbfv_context *c, const uint64_t *x_coeff, long x_stride, uint64_t *out,
                        int batch, uint64_t *tmp, hipStream_t st);

@@ -254,7 +254,120 @@ __global__ void k_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint
     out[(rest * qn + l) * (long)n + k] = v;
 }
 
+// modUp T -> PQ and modDown PQ -> Q fused (the two kernels above, composed):
+// in [batch][2][ri_blocks][tn][N] coeff -> out [batch][2][qn][N] (+ add), with
+// one thread per (coefficient, ciphertext-poly).  The PQ limbs come out of the
+// ri blocks in index order, P limbs first, so the thread keeps the PN special
+// residues in registers and finishes each Q limb's modDown as soon as its
+// modUp value exists: the [2][pn+qn] PQ polynomials never reach memory.  The
+// arithmetic of every limb is that of k_modup_t_to_pq then k_moddown.  TN, G
+// and PN (tn, gamma, pn) are compile-time so that the per-thread arrays stay
+// in registers.  grid (n / KS_BS, batch * 2)
+template <int TN, int G, int PN>
+__global__ void k_modup_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                const uint64_t *__restrict__ add, long add_stride, int qn, int B, int oQ,
+                                int oT, int n) {
+    const int k = blockIdx.x * KS_BS + threadIdx.x;
+    const long rest = blockIdx.y;
+    const int h = (int)(rest & 1);
+    const long bt = rest >> 1;
+    const int pq = PN + qn;
+    uint64_t xp[PN];  // the P residues of this coefficient
+    // the T values of ri block b + 1 are loaded while block b is converted
+    const uint64_t *ip0 = in + rest * B * (long)TN * n + k;
+    uint64_t nxt[TN];
+#pragma unroll
+    for (int kk = 0; kk < TN; ++kk) nxt[kk] = ip0[(long)kk * n];
+    for (int b = 0; b < B; ++b) {
+        uint64_t cur[TN];
+#pragma unroll
+        for (int kk = 0; kk < TN; ++kk) cur[kk] = nxt[kk];
+        if (b + 1 < B) {
+#pragma unroll
+            for (int kk = 0; kk < TN; ++kk) nxt[kk] = ip0[((long)(b + 1) * TN + kk) * n];
+        }
+        uint64_t rb[TN];
+        float vv = 0;
+#pragma unroll
+        for (int kk = 0; kk < TN; ++kk) {
+            const uint64_t tm = t.mod[oT + kk];
+            const uint64_t ra = cur[kk] + t.halfT_mod[pq + kk];
+            const kbfv_tw hw = t.t_hatinv[kk];
+            rb[kk] = kb_shoup(ra, hw.w, hw.ws, tm);
+            const float rr = (float)rb[kk];
+            vv += rr / (float)tm;
+        }
+        const int vf = (int)vv;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int idx = b * G + j;
+            if (idx >= pq) break;
+            const uint64_t r = t.mod[idx];
+            kbfv_u128 acc{0, 0};
+#pragma unroll
+            for (int kk = 0; kk < TN; ++kk) kb_mac128(acc, rb[kk], t.t_hat_mod_pq[idx * TN + kk]);
+            kb_sub128(acc, (uint64_t)vf * t.t_mod_pq[idx]);
+            uint64_t v = kb_barrett128(acc, r, t.mu[2 * idx], t.mu[2 * idx + 1]);
+            v = kb_csub(v + r - t.halfT_mod[idx], r);
+            if (idx < PN) {
+#pragma unroll
+                for (int jj = 0; jj < PN; ++jj)
+                    if (jj == idx) xp[jj] = v;
+                continue;
+            }
+            // modDown of Q limb l from its modUp value v and the P residues
+            const int l = idx - PN;
+            const int m = oQ + l;
+            const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+            uint64_t ra;
+            if constexpr (PN == 1) {
+                ra = kb_barrett128(kbfv_u128{xp[0], 0}, q, mlo, mhi);
+            } else {
+                kbfv_u128 a2{0, 0};
+#pragma unroll
+                for (int jj = 0; jj < PN; ++jj) {
+                    const kbfv_tw hi = t.p_hatinv[jj];
+                    const uint64_t rp = kb_shoup(xp[jj], hi.w, hi.ws, t.mod[jj]);
+                    kb_mac128(a2, rp, t.p_hat_mod_q[l * PN + jj]);
+                }
+                ra = kb_barrett128(a2, q, mlo, mhi);
+            }
+            ra = q - ra + v;
+            const kbfv_tw pi = t.Pinv_mod_q[l];
+            uint64_t o = kb_shoup(ra, pi.w, pi.ws, q);
+            if (add) o = kb_addmod(o, add[bt * add_stride + ((long)h * qn + l) * n + k], q);
+            out[(rest * qn + l) * (long)n + k] = o;
+        }
+    }
+}
+
 }  // namespace
+
+bool kb_modup_moddown_fused(const kbfv_context *c) {
+    return c->pn == 1 && ((c->tn == 4 && c->gamma == 2) || (c->tn == 3 && c->gamma == 1));
+}
+
+int kb_modup_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
+                     long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st) {
+    KB_REQUIRE(c && in && out && batch > 0, "bad modUp/modDown arguments");
+    KB_REQUIRE(2L * batch <= 65535, "batch too large for one modUp/modDown launch");
+    // PN special residues first: the P limbs must all lie before any Q limb
+#define KB_MUMD(TN_, G_, PN_)                                                                              \
+    if (c->tn == TN_ && c->gamma == G_ && c->pn == PN_) {                                                  \
+        hipLaunchKernelGGL((k_modup_moddown<TN_, G_, PN_>), dim3(c->n / KS_BS, 2 * batch), dim3(KS_BS), 0, \
+                           st, c->t, in, out, add, add_stride, c->qn, c->ri_blocks, c->oQ, c->oT, c->n);    \
+        KB_LAUNCH_CHECK();                                                                                 \
+        return KBFV_OK;                                                                                    \
+    }
+    KB_MUMD(4, 2, 1)
+    KB_MUMD(3, 1, 1)
+#undef KB_MUMD
+    // other shapes: the two kernels through the PQ buffer
+    KB_REQUIRE(pq_tmp, "modUp/modDown of this shape needs a PQ buffer");
+    const int rc = kb_modup_t_to_pq(c, in, pq_tmp, batch, st);
+    if (rc != KBFV_OK) return rc;
+    return kb_moddown(c, pq_tmp, out, add, add_stride, batch, st);
+}
 
 int kb_modup_qj_to_t(const kbfv_context *c, const uint64_t *x, long xs, uint64_t *out, int batch,
                      hipStream_t st) {

@@ -248,19 +248,19 @@ struct MultWs {
     uint64_t *CT;           // digits over T         [cb][dnum][tn][N]     (region C, later)
     uint64_t *MD;           // modDown out           [cb][2][qn][N]        (region C, later)
     uint64_t *IP;           // inner product         [cb][2][B][tn][N]     (region E)
-    uint64_t *PQ;           // T -> PQ               [cb][2][pq][N]        (region F)
+    uint64_t *PQ;           // T -> PQ, only for shapes without the fused modUp/modDown (region F)
 };
 
 size_t mult_layout(const kbfv_context *c, int cb, void *base, MultWs *w) {
-    const size_t N = c->n, qn = c->qn, rn = c->rn, pq = c->pn + c->qn;
+    const size_t N = c->n, qn = c->qn, rn = c->rn;
     const size_t A = std::max(4 * qn, 3 * rn) * N;
     const size_t Bq = std::max(4 * rn, 3 * qn) * N;
     const size_t Cc = std::max(std::max(3 * (qn + rn), (size_t)c->dnum * c->tn), 2 * qn) * N;
     const size_t E = 2 * (size_t)c->ri_blocks * c->tn * N;
-    const size_t F = 2 * pq * N;
+    const size_t F = kb_modup_moddown_fused(c) ? 0 : 2 * (size_t)(c->pn + c->qn) * N;
     Bump b(base);
     uint64_t *a = b.take(A * cb), *bb = b.take(Bq * cb), *cc = b.take(Cc * cb);
-    uint64_t *e = b.take(E * cb), *f = b.take(F * cb);
+    uint64_t *e = b.take(E * cb), *f = F ? b.take(F * cb) : nullptr;
     if (w) {
         w->X1 = a;
         w->X2 = a ? a + 2 * qn * N * cb : nullptr;
@@ -279,14 +279,14 @@ size_t mult_layout(const kbfv_context *c, int cb, void *base, MultWs *w) {
 
 int mult_chunk(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1, const uint64_t *ct2,
                uint64_t *out, int cb, const MultWs &w, hipStream_t st) {
-    const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn, pq = c->pn + c->qn;
+    const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn;
     const long CTS = 2 * qn * N;  // ciphertext stride
     // 1. INTT of both inputs (out of place; the NTT-domain inputs stay for the tensor)
     KB_TRY(kb_ntt(c, -1, ct1, w.X1, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
     KB_TRY(kb_ntt(c, -1, ct2, w.X2, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
     // 2. HPS base extension Q -> R, 3. NTT over R
-    KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 2 * cb, st));
-    KB_TRY(kb_bconv(c, true, w.X2, qn * N, w.R2, rn * N, 2 * cb, st));
+    // (X2 follows X1 and R2 follows R1: one launch covers both ciphertexts)
+    KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * cb, st));
     KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * cb, 1, 0, rn * N, N, 0, rn * N, N, st));
     // 4. tensor over QR, 5. INTT over QR
     KB_TRY(kb_tensor_qr(c, ct1, ct2, w.R1, w.R2, w.D3, cb, st));
@@ -300,11 +300,9 @@ int mult_chunk(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1, 
     KB_TRY(kb_inner_product(c, w.CT, rlk, w.IP, cb, st));
     KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * cb, 1, 0, (long)c->tn * N, N, 0,
                   (long)c->tn * N, N, st));
-    // 11. T -> PQ, 12. modDown with the a1b2+a2b1 and b1b2 terms added
-    KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, cb, st));
-    KB_TRY(kb_moddown(c, w.PQ, w.MD, w.DQ + qn * N, 3 * qn * N, cb, st));
+    // 11. T -> PQ and 12. modDown with the a1b2+a2b1 and b1b2 terms added, in one kernel
+    KB_TRY(kb_modup_moddown(c, w.IP, w.MD, w.DQ + qn * N, 3 * qn * N, cb, w.PQ, st));
     // 13. back to the NTT domain, into the output ciphertexts
-    (void)pq;
     return kb_ntt(c, +1, w.MD, out, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st);
 }
 
@@ -575,8 +573,7 @@ extern "C" int kbfv_mult_after_ip(const kbfv_context *c, const uint64_t *ip_out,
         // steps 10-13 of the fused pipeline; the caller's inner-product output is left as it is
         KB_TRY(kb_ntt(c, -1, ip_out + b0 * IPO, w.IP, c->oT, c->tn, 2 * c->ri_blocks * n, 1, 0,
                       (long)c->tn * N, N, 0, (long)c->tn * N, N, s));
-        KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, n, s));
-        KB_TRY(kb_moddown(c, w.PQ, w.MD, d01 + b0 * CTS, CTS, n, s));
+        KB_TRY(kb_modup_moddown(c, w.IP, w.MD, d01 + b0 * CTS, CTS, n, w.PQ, s));
         KB_TRY(kb_ntt(c, +1, w.MD, out + b0 * CTS, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
     }
     return KBFV_OK;
@@ -664,15 +661,15 @@ extern "C" int kbfv_keygen_galois(const kbfv_context *c, const uint8_t seed[32],
 }
 
 struct RotWs {
-    uint64_t *X, *CT, *IP, *PQ, *MD;
+    uint64_t *X, *CT, *IP, *MD, *PQ;
 };
 static size_t rot_layout(const kbfv_context *c, int cb, void *base, RotWs *w) {
     const size_t N = c->n, qn = c->qn, pq = c->pn + c->qn;
     Bump bp(base);
     uint64_t *x = bp.take(2 * qn * N * cb), *ct = bp.take((size_t)c->dnum * c->tn * N * cb),
-             *ip = bp.take(2 * (size_t)c->ri_blocks * c->tn * N * cb), *pqb = bp.take(2 * pq * N * cb),
-             *md = bp.take(2 * qn * N * cb);
-    if (w) *w = RotWs{x, ct, ip, pqb, md};
+             *ip = bp.take(2 * (size_t)c->ri_blocks * c->tn * N * cb), *md = bp.take(2 * qn * N * cb);
+    uint64_t *pqb = kb_modup_moddown_fused(c) ? nullptr : bp.take(2 * pq * N * cb);
+    if (w) *w = RotWs{x, ct, ip, md, pqb};
     return bp.used;
 }
 
@@ -699,8 +696,7 @@ extern "C" int kbfv_rotate(const kbfv_context *c, const uint64_t *gkey, uint64_t
         KB_TRY(kb_inner_product(c, w.CT, gkey, w.IP, nb, s));
         KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * nb, 1, 0, (long)c->tn * N, N, 0,
                       (long)c->tn * N, N, s));
-        KB_TRY(kb_modup_t_to_pq(c, w.IP, w.PQ, nb, s));
-        KB_TRY(kb_moddown(c, w.PQ, w.MD, nullptr, 0, nb, s));
+        KB_TRY(kb_modup_moddown(c, w.IP, w.MD, nullptr, 0, nb, w.PQ, s));
         for (int b = 0; b < nb; ++b)  // KS(a) + (0, b): b joins the second polynomial only
             KB_TRY(kb_poly_binop(c, 0, w.MD + b * CTS + qn * N, w.X + b * CTS + qn * N, w.MD + b * CTS + qn * N,
                                  c->oQ, qn, 1, s));
