@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # runs over `bench.py --roofline-only`, each scaled by the factor the same
 # counter reads for a 4 GiB device copy (fetch 2.000, write 1.000); recorded
 # in profiles/ntt_qr_inverse_n65536_b128_counters.md.
-NTT_TRAFFIC_BYTES = {(16, 28, 128): 22_982_288_517}
+NTT_TRAFFIC_BYTES = {(16, 28, 128): 22_991_137_126}
 
 
 def parse():
