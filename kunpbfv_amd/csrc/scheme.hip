@@ -511,11 +511,13 @@ extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const
     int cb = batch;
     while (cb > 1 && mult_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
     KB_REQUIRE(mult_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
-    MultWs w;
-    mult_layout(c, cb, ws, &w);
     const long CTS = 2L * c->qn * c->n;
     for (int b0 = 0; b0 < batch; b0 += cb) {
         const int n = std::min(cb, batch - b0);
+        // laid out for this chunk's own size: the pipeline treats the second
+        // input's polynomials (X2, R2) as following the first's directly
+        MultWs w;
+        mult_layout(c, n, ws, &w);
         KB_TRY(mult_chunk(c, rlk, ct1 + b0 * CTS, ct2 + b0 * CTS, out + b0 * CTS, n, w, S(st)));
     }
     return KBFV_OK;
@@ -531,19 +533,18 @@ extern "C" int kbfv_mult_before_ip(const kbfv_context *c, const uint64_t *ct1, c
     int cb = batch;
     while (cb > 1 && mult_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
     KB_REQUIRE(mult_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
-    MultWs w;
-    mult_layout(c, cb, ws, &w);
     const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn, CTS = 2 * qn * N;
     const long IPS = (long)c->dnum * c->tn * N;
     hipStream_t s = S(st);
     for (int b0 = 0; b0 < batch; b0 += cb) {
         const int n = std::min(cb, batch - b0);
+        MultWs w;  // per chunk size, as in kbfv_mult_relin
+        mult_layout(c, n, ws, &w);
         const uint64_t *a = ct1 + b0 * CTS, *b = ct2 + b0 * CTS;
         // steps 1-8 of the fused pipeline (mult_chunk), the digits going to ip_in
         KB_TRY(kb_ntt(c, -1, a, w.X1, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
         KB_TRY(kb_ntt(c, -1, b, w.X2, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
-        KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 2 * n, s));
-        KB_TRY(kb_bconv(c, true, w.X2, qn * N, w.R2, rn * N, 2 * n, s));
+        KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * n, s));
         KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * n, 1, 0, rn * N, N, 0, rn * N, N, s));
         KB_TRY(kb_tensor_qr(c, a, b, w.R1, w.R2, w.D3, n, s));
         KB_TRY(kb_ntt(c, -1, w.D3, w.D3, c->oQ, sn, 3 * n, 1, 0, sn * N, N, 0, sn * N, N, s));
