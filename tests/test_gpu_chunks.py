@@ -3,7 +3,8 @@ and the last chunk may be shorter than the others.  Every chunk must lay out
 its buffers for its own size (the pipeline takes the second input's
 polynomials as following the first's directly), so results with a workspace
 for two ciphertexts on batches of 3 and 5 (chunks 2+1 and 2+2+1) must equal the
-unchunked ones bit for bit, for kbfv_mult_relin and for the split-IP seam."""
+unchunked ones bit for bit, for kbfv_mult_relin, the split-IP seam and
+kbfv_rotate."""
 import numpy as np
 import pytest
 
@@ -51,4 +52,20 @@ def test_short_last_chunk_split_ip(setup, batch):
     ip_in, d01 = ctx.mult_before_ip(c1, c2, ws=ws)
     ip_out = ctx.ip_batch(ip_in, rlk)
     out = ctx.mult_after_ip(ip_out, d01, ws=ws)
+    assert np.array_equal(kb.to_host(out), full)
+
+
+@pytest.mark.parametrize("batch", [3, 5])
+def test_short_last_chunk_rotate(setup, batch):
+    import torch
+    kb, ctx, rlk, ct = setup
+    sk = ctx.keygen_sk(bytes([41]) * 32)
+    g = ctx.galois_elt(1)
+    gk = ctx.keygen_galois(bytes([46]) * 32, sk, g)
+    c1 = ct[:batch].contiguous()
+    full = kb.to_host(ctx.rotate(gk, g, c1))
+    out = ctx.empty(*c1.shape)
+    ws = torch.empty(kb.lib.kbfv_rotate_workspace_bytes(ctx.h, 2), dtype=torch.uint8, device="cuda")
+    kb.check(kb.lib.kbfv_rotate(ctx.h, kb.ptr(gk), g, kb.ptr(c1), kb.ptr(out), batch, kb.ptr(ws), ws.numel(),
+                                kb.stream_ptr(None)))
     assert np.array_equal(kb.to_host(out), full)
