@@ -324,7 +324,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     // twiddles: bit-reversed psi powers, n^-1 folded into psiinv[1]
     // (include/Utils.cuh:171-191)
     c->psi.assign(c->nmod, 0);
-    std::vector<kbfv_tw> tw((size_t)c->nmod * 2 * N), ninv(c->nmod);
+    std::vector<kbfv_tw> tw((size_t)c->nmod * 2 * N), ninv(c->nmod), inv_w1(c->nmod);
     for (int m = 0; m < c->nmod; m++) {
         if (m == c->oPlain && !c->batching) continue;
         uint64_t qm = mod[m];
@@ -339,6 +339,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
             f[nx] = mulmod(psi, f[pr], qm);
             iv[nx] = mulmod(psiinv, iv[pr], qm);
         }
+        inv_w1[m] = tw_of(iv[1], qm);  // the last inverse stage's twiddle without n^-1
         iv[1] = mulmod(iv[1], ni, qm);
         kbfv_tw *base = tw.data() + (size_t)m * 2 * N;
         for (int i = 0; i < N; i++) base[i] = tw_of(f[i], qm), base[N + i] = tw_of(iv[i], qm);
@@ -461,6 +462,14 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
         half.divmod(2);
         for (int i = 0; i < pq; i++) halfT[i] = half.mod(PQ[i]);
         for (int k = 0; k < tn; k++) halfT[pq + k] = half.mod(T[k]);
+    }
+    // the same constants for inputs that are N times the INTT (its n^-1 left
+    // out): [q_hatinv n^-1]_q, [t_hatinv n^-1]_t and [N halfT]_t
+    std::vector<uint64_t> q_hatinv_n(qn), t_hatinv_n(tn), halfT_n(tn);
+    for (int i = 0; i < qn; i++) q_hatinv_n[i] = mulmod(q_hatinv[i], invmod((uint64_t)N % Q[i], Q[i]), Q[i]);
+    for (int k = 0; k < tn; k++) {
+        t_hatinv_n[k] = mulmod(t_hatinv[k], invmod((uint64_t)N % T[k], T[k]), T[k]);
+        halfT_n[k] = mulmod(halfT[pq + k], (uint64_t)N % T[k], T[k]);
     }
     std::vector<uint64_t> p_hat_mod_q((size_t)qn * pn), p_hatinv(pn), P_mod_q(qn), Pinv(qn);
     for (int l = 0; l < qn; l++)
@@ -595,6 +604,8 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     size_t o_pinv = blob.add(tws(Pinv, Q)), o_pmq = blob.add(P_mod_q);
     size_t o_mxqr = blob.add(mx_qr), o_mxrq = blob.add(mx_rq), o_mxsr = blob.add(mx_sr);
     size_t o_seal = blob.add(seal_map);
+    size_t o_qhin = blob.add(tws(q_hatinv_n, Q)), o_thn = blob.add(tws(t_hatinv_n, T)), o_hfn = blob.add(halfT_n);
+    size_t o_iw1 = blob.add(inv_w1);
 
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipMalloc(&c->d_blob, blob.bytes.size());
@@ -629,6 +640,8 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     d.p_hat_mod_q = (const uint64_t *)at(o_phq), d.p_hatinv = (const kbfv_tw *)at(o_phi);
     d.Pinv_mod_q = (const kbfv_tw *)at(o_pinv), d.P_mod_q = (const uint64_t *)at(o_pmq);
     if (c->batching) d.seal_map = (const uint64_t *)at(o_seal);
+    d.q_hatinv_n = (const kbfv_tw *)at(o_qhin), d.t_hatinv_n = (const kbfv_tw *)at(o_thn);
+    d.halfT_n = (const uint64_t *)at(o_hfn), d.inv_w1 = (const kbfv_tw *)at(o_iw1);
     if (mx_ok) {
         d.mx_qr = (const uint64_t *)at(o_mxqr), d.mx_rq = (const uint64_t *)at(o_mxrq);
         d.mx_sr = (const uint64_t *)at(o_mxsr);

@@ -87,6 +87,11 @@ struct kb_dev_tables {
     const uint64_t *mx_sr;  // scale-and-round, K = qn
     // SEAL slot order: NTT index of slot i (batching only, else null)
     const uint64_t *seal_map;  // [N]
+    // for inputs that are N times the INTT (kb_intt_noscale): n^-1 folded in
+    const kbfv_tw *q_hatinv_n;  // [qn] [q_hatinv n^-1]_q
+    const kbfv_tw *t_hatinv_n;  // [tn] [t_hatinv n^-1]_t
+    const uint64_t *halfT_n;    // [tn] [N (T/2 mod t)]_t
+    const kbfv_tw *inv_w1;      // [nmod] inverse table entry 1 without its n^-1
 };
 
 struct kbfv_context {
@@ -120,6 +125,10 @@ static inline uint64_t kb_host_mulmod(uint64_t a, uint64_t b, uint64_t q) {
 int kb_ntt(const kbfv_context *c, int dir, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
            int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
            hipStream_t st);
+// inverse NTT without the final n^-1 and reduction: dst = N * INTT(src) as lazy
+// values in [0, 4q), for consumers that fold n^-1 into their first constant
+int kb_intt_noscale(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs, int polys,
+                    int batch, long sB, long sY, long sL, long dB, long dY, long dL, hipStream_t st);
 // contiguous [batch*polys][limbs][N] in place
 int kb_ntt_inplace(const kbfv_context *c, int dir, uint64_t *data, int mod0, int limbs, int polys,
                    hipStream_t st);

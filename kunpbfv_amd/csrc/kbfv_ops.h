@@ -5,8 +5,10 @@
 // rns.hip
 int kb_poly_binop(const kbfv_context *c, int op, const uint64_t *a, const uint64_t *b, uint64_t *o,
                   int mod0, int limbs, int polys, hipStream_t st);
+// noscale: x is N times the canonical input (kb_intt_noscale output, Q -> R
+// only); the n^-1 rides in the first constant
 int kb_bconv(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stride, uint64_t *y,
-             long out_stride, int polys, hipStream_t st);
+             long out_stride, int polys, hipStream_t st, bool noscale = false);
 int kb_tensor_qr(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, const uint64_t *r1,
                  const uint64_t *r2, uint64_t *out, int batch, hipStream_t st);
 int kb_tensor(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out, int mod0,
@@ -20,7 +22,7 @@ int kb_reduce_plain(const kbfv_context *c, uint64_t *v, long count, hipStream_t 
 // cores; return 1 (nothing launched) when the bases are too large for them
 bool kb_rns_gemm_fits(int k_rows, int out_n);
 int kb_bconv_mfma(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stride, uint64_t *y,
-                  long out_stride, int polys, hipStream_t st);
+                  long out_stride, int polys, hipStream_t st, bool noscale = false);
 int kb_scale_round_mfma(const kbfv_context *c, const uint64_t *x, uint64_t *y, int polys, hipStream_t st);
 
 // ks.hip
@@ -42,6 +44,12 @@ int kb_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const u
 bool kb_modup_moddown_fused(const kbfv_context *c);
 int kb_modup_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
                      long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st);
+// the key-switch tail: inner-product output ip [batch][2][ri_blocks][tn][N]
+// (NTT over T, left as it is) -> INTT over T into ip_coeff (same shape) ->
+// modUp T->PQ and modDown into out [batch][2][qn][N] (+ add).  For the fused
+// shapes the INTT leaves out n^-1 and the modUp folds it into its constants.
+int kb_ks_tail(const kbfv_context *c, const uint64_t *ip, uint64_t *ip_coeff, uint64_t *out, const uint64_t *add,
+               long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st);
 // digit decomposition of the Q limbs into T followed by the forward NTT over T
 int kb_digits_to_t_ntt(const kbfv_context *c, const uint64_t *x_coeff, long x_stride, uint64_t *out,
                        int batch, uint64_t *tmp, hipStream_t st);

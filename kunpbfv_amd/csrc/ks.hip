@@ -262,8 +262,11 @@ __global__ void k_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint
 // modUp value exists: the [2][pn+qn] PQ polynomials never reach memory.  The
 // arithmetic of every limb is that of k_modup_t_to_pq then k_moddown.  TN, G
 // and PN (tn, gamma, pn) are compile-time so that the per-thread arrays stay
-// in registers.  grid (n / KS_BS, batch * 2)
-template <int TN, int G, int PN>
+// in registers.  NOSC: the input is N times the INTT (lazy, below 4t); the
+// first step [(x + T/2) (T/t)^-1]_t is a Shoup product exact for any 64-bit
+// operand, so it takes ([N T/2]_t, [(T/t)^-1 n^-1]_t) instead and gives the
+// same canonical value.  grid (n / KS_BS, batch * 2)
+template <int TN, int G, int PN, bool NOSC>
 __global__ void k_modup_moddown(kb_dev_tables t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                 const uint64_t *__restrict__ add, long add_stride, int qn, int B, int oQ,
                                 int oT, int n) {
@@ -291,8 +294,8 @@ __global__ void k_modup_moddown(kb_dev_tables t, const uint64_t *__restrict__ in
 #pragma unroll
         for (int kk = 0; kk < TN; ++kk) {
             const uint64_t tm = t.mod[oT + kk];
-            const uint64_t ra = cur[kk] + t.halfT_mod[pq + kk];
-            const kbfv_tw hw = t.t_hatinv[kk];
+            const uint64_t ra = cur[kk] + (NOSC ? t.halfT_n[kk] : t.halfT_mod[pq + kk]);
+            const kbfv_tw hw = NOSC ? t.t_hatinv_n[kk] : t.t_hatinv[kk];
             rb[kk] = kb_shoup(ra, hw.w, hw.ws, tm);
             const float rr = (float)rb[kk];
             vv += rr / (float)tm;
@@ -347,26 +350,46 @@ bool kb_modup_moddown_fused(const kbfv_context *c) {
     return c->pn == 1 && ((c->tn == 4 && c->gamma == 2) || (c->tn == 3 && c->gamma == 1));
 }
 
-int kb_modup_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
-                     long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st) {
+static int modup_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
+                         long add_stride, int batch, uint64_t *pq_tmp, bool nosc, hipStream_t st) {
     KB_REQUIRE(c && in && out && batch > 0, "bad modUp/modDown arguments");
     KB_REQUIRE(2L * batch <= 65535, "batch too large for one modUp/modDown launch");
     // PN special residues first: the P limbs must all lie before any Q limb
-#define KB_MUMD(TN_, G_, PN_)                                                                              \
-    if (c->tn == TN_ && c->gamma == G_ && c->pn == PN_) {                                                  \
-        hipLaunchKernelGGL((k_modup_moddown<TN_, G_, PN_>), dim3(c->n / KS_BS, 2 * batch), dim3(KS_BS), 0, \
-                           st, c->t, in, out, add, add_stride, c->qn, c->ri_blocks, c->oQ, c->oT, c->n);    \
+#define KB_MUMD(TN_, G_, PN_, NOSC_)                                                                       \
+    if (c->tn == TN_ && c->gamma == G_ && c->pn == PN_ && nosc == NOSC_) {                                 \
+        hipLaunchKernelGGL((k_modup_moddown<TN_, G_, PN_, NOSC_>), dim3(c->n / KS_BS, 2 * batch), dim3(KS_BS), \
+                           0, st, c->t, in, out, add, add_stride, c->qn, c->ri_blocks, c->oQ, c->oT, c->n); \
         KB_LAUNCH_CHECK();                                                                                 \
         return KBFV_OK;                                                                                    \
     }
-    KB_MUMD(4, 2, 1)
-    KB_MUMD(3, 1, 1)
+    KB_MUMD(4, 2, 1, false)
+    KB_MUMD(3, 1, 1, false)
+    KB_MUMD(4, 2, 1, true)
+    KB_MUMD(3, 1, 1, true)
 #undef KB_MUMD
+    KB_REQUIRE(!nosc, "unscaled modUp input needs a fused shape");
     // other shapes: the two kernels through the PQ buffer
     KB_REQUIRE(pq_tmp, "modUp/modDown of this shape needs a PQ buffer");
     const int rc = kb_modup_t_to_pq(c, in, pq_tmp, batch, st);
     if (rc != KBFV_OK) return rc;
     return kb_moddown(c, pq_tmp, out, add, add_stride, batch, st);
+}
+
+int kb_modup_moddown(const kbfv_context *c, const uint64_t *in, uint64_t *out, const uint64_t *add,
+                     long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st) {
+    return modup_moddown(c, in, out, add, add_stride, batch, pq_tmp, false, st);
+}
+
+int kb_ks_tail(const kbfv_context *c, const uint64_t *ip, uint64_t *ip_coeff, uint64_t *out, const uint64_t *add,
+               long add_stride, int batch, uint64_t *pq_tmp, hipStream_t st) {
+    KB_REQUIRE(c && ip && ip_coeff && out && batch > 0, "bad key-switch arguments");
+    const long N = c->n, TS = (long)c->tn * N;
+    const int polys = 2 * c->ri_blocks * batch;
+    const bool nosc = kb_modup_moddown_fused(c);
+    const int rc = nosc ? kb_intt_noscale(c, ip, ip_coeff, c->oT, c->tn, polys, 1, 0, TS, N, 0, TS, N, st)
+                        : kb_ntt(c, -1, ip, ip_coeff, c->oT, c->tn, polys, 1, 0, TS, N, 0, TS, N, st);
+    if (rc) return rc;
+    return modup_moddown(c, ip_coeff, out, add, add_stride, batch, pq_tmp, nosc, st);
 }
 
 int kb_modup_qj_to_t(const kbfv_context *c, const uint64_t *x, long xs, uint64_t *out, int batch,

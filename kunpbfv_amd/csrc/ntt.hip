@@ -137,7 +137,7 @@ __device__ __forceinline__ void round_twiddles(kbfv_tw *tws, const int lam0, con
 // 128-byte runs); it writes them back to the tile or, when GOUT, straight to
 // global memory with the final reduction of the last pass (FIN).
 template <int LR, int LOG_S, bool COLUMN, bool FWD, bool NINV, bool GIN = false, bool GOUT = false,
-          bool FIN = false, bool PRE = false>
+          bool FIN = false, bool PRE = false, bool W1 = false>
 __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const kbfv_tw *tab,
                                           const int log_n, const int g0, const int grp0,
                                           const int stride, const uint64_t q, const kbfv_tw ninv,
@@ -207,6 +207,8 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
                         w = (*pre)[(1 << j) - 1 + m];
                     else
                         w = tab[(1 << gst) + (pos >> (log_n - gst))];
+                    // W1: stage 0 takes the unscaled entry 1, passed in ninv
+                    if (W1 && gst == 0) w = ninv;
 #pragma unroll
                     for (int i = i0; i < i0 + h; ++i) {
                         gs_bfly(x[i], x[i + h], w, nq, q4, nq4);
@@ -259,12 +261,12 @@ __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restr
 template <int LOG_N, bool COLUMN>
 constexpr int pass_log() { return COLUMN ? (LOG_N + 1) / 2 : LOG_N - (LOG_N + 1) / 2; }
 
-template <int LOG_N, bool COLUMN, bool FWD>
+template <int LOG_N, bool COLUMN, bool FWD, bool NOSCALE = false>
 __global__ void __launch_bounds__(TILE >> LOG_R)
     k_ntt_pass(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
                const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
-               const uint64_t *__restrict__ mods, int mod0, int limbs, int polys,
-               int batch, long sB, long sY, long sL, long dB, long dY, long dL) {
+               const kbfv_tw *__restrict__ w1_tab, const uint64_t *__restrict__ mods, int mod0, int limbs,
+               int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL) {
     constexpr int LOG_S = pass_log<LOG_N, COLUMN>();
     constexpr int log_n = LOG_N;
     using G = Geo<LOG_S, COLUMN>;
@@ -298,8 +300,10 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     constexpr int LR0 = LOG_S < LOG_R ? LOG_S : LOG_R;
     constexpr int LR1 = LOG_S - LR0;
     static_assert(LR1 > 0 && LR1 <= LOG_R, "two register rounds per pass");
-    constexpr bool NI = (!FWD && LAST);
-    const kbfv_tw ninv = NI ? ninv_tab[m] : kbfv_tw{0, 0};
+    // NOSCALE (inverse column pass): N x INTT, lazy, stage 0 with the
+    // unscaled twiddle (passed as w1_tab) and no n^-1 multiply
+    constexpr bool NI = (!FWD && LAST && !NOSCALE);
+    const kbfv_tw ninv = NI ? ninv_tab[m] : (NOSCALE ? w1_tab[m] : kbfv_tw{0, 0});
     // the forward pass's first register round (stride S/16) and the inverse pass's last
     // touch 128-byte runs in both layouts; the stride-1 rounds of the row pass
     // go through the LDS tile for both their global load and store.
@@ -336,8 +340,8 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
                 ntt_round<LR1, LOG_S, COLUMN, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
             }
         }
-        ntt_round<LR0, LOG_S, COLUMN, false, NI, false, true, LAST>(tile, 0, tab, log_n, g0, grp0, stride, q,
-                                                                    ninv, nullptr, out);
+        ntt_round<LR0, LOG_S, COLUMN, false, NI, false, true, LAST && !NOSCALE, false, NOSCALE>(
+            tile, 0, tab, log_n, g0, grp0, stride, q, ninv, nullptr, out);
         return;
     }
 
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     }
 }
 
-template <int LOG_N, bool COLUMN, bool FWD>
+template <int LOG_N, bool COLUMN, bool FWD, bool NOSCALE = false>
 int launch_pass(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
                 int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
                 hipStream_t st) {
@@ -364,18 +368,19 @@ int launch_pass(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int m
     const long groups = (1L << (LOG_N - pass_log<LOG_N, COLUMN>())) / G::C;
     const long blocks = groups * limbs * polys * batch;
     KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
-    hipLaunchKernelGGL((k_ntt_pass<LOG_N, COLUMN, FWD>), dim3((unsigned)blocks), dim3(G::T), 0, st, src, dst,
-                       c->t.tw, c->t.ninv, c->t.mod, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL);
+    hipLaunchKernelGGL((k_ntt_pass<LOG_N, COLUMN, FWD, NOSCALE>), dim3((unsigned)blocks), dim3(G::T), 0, st, src, dst,
+                       c->t.tw, c->t.ninv, c->t.inv_w1, c->t.mod, mod0, limbs, polys, batch, sB, sY, sL, dB, dY,
+                       dL);
     KB_LAUNCH_CHECK();
     return KBFV_OK;
 }
 
-template <bool COLUMN, bool FWD>
+template <bool COLUMN, bool FWD, bool NOSCALE = false>
 int launch_pass_n(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs, int polys,
                   int batch, long sB, long sY, long sL, long dB, long dY, long dL, hipStream_t st) {
 #define KB_NTT_N(LN)                                                                                     \
     case LN:                                                                                             \
-        return launch_pass<LN, COLUMN, FWD>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, \
+        return launch_pass<LN, COLUMN, FWD, NOSCALE>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, \
                                             dL, st);
     switch (c->log_n) {
         KB_NTT_N(12)
@@ -417,4 +422,15 @@ int kb_ntt_inplace(const kbfv_context *c, int dir, uint64_t *data, int mod0, int
     const long N = c->n;
     return kb_ntt(c, dir, data, data, mod0, limbs, polys, 1, 0, (long)limbs * N, N, 0,
                   (long)limbs * N, N, st);
+}
+
+int kb_intt_noscale(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs, int polys,
+                    int batch, long sB, long sY, long sL, long dB, long dY, long dL, hipStream_t st) {
+    KB_REQUIRE(c && src && dst, "null argument");
+    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
+    KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod, "modulus range out of bounds");
+    KB_REQUIRE(polys > 0 && batch > 0, "empty NTT");
+    const int rc = launch_pass_n<false, false>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
+    if (rc) return rc;
+    return launch_pass_n<true, false, true>(c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, st);
 }

@@ -288,19 +288,22 @@ int kb_poly_binop(const kbfv_context *c, int op, const uint64_t *a, const uint64
 }
 
 int kb_bconv(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stride, uint64_t *y,
-             long out_stride, int polys, hipStream_t st) {
+             long out_stride, int polys, hipStream_t st, bool noscale) {
     KB_REQUIRE(c && x && y && polys > 0, "bad bconv arguments");
     KB_REQUIRE(c->rn <= BC_MAXIN, "too many limbs for base conversion");
+    KB_REQUIRE(q_to_r || !noscale, "unscaled input only for Q -> R");
     if (c->t.mx_qr && kb_rns_gemm_fits((q_to_r ? c->qn : c->rn) + 1, q_to_r ? c->rn : c->qn)) {
-        const int r = kb_bconv_mfma(c, q_to_r, x, in_stride, y, out_stride, polys, st);
+        const int r = kb_bconv_mfma(c, q_to_r, x, in_stride, y, out_stride, polys, st, noscale);
         if (r != 1) return r;
     }
     const int tiles = c->n / BC_TILE;
     const kb_dev_tables &t = c->t;
+    // y_i = [x_i (Q/q_i)^-1]_q_i is a Shoup product, exact for any 64-bit x_i:
+    // an unscaled input takes the constant with n^-1 folded in
     if (q_to_r)
         hipLaunchKernelGGL(k_bconv, dim3(tiles * polys), dim3(BC_TILE * BC_WAVES), 0, st, x, in_stride, y,
-                           out_stride, t.mod, t.mu, t.q_hatinv, t.q_hat_mod_r, t.alphaQ_mod_r, t.q_inv,
-                           c->oQ, c->qn, c->oR, c->rn, c->n);
+                           out_stride, t.mod, t.mu, noscale ? t.q_hatinv_n : t.q_hatinv, t.q_hat_mod_r,
+                           t.alphaQ_mod_r, t.q_inv, c->oQ, c->qn, c->oR, c->rn, c->n);
     else
         hipLaunchKernelGGL(k_bconv, dim3(tiles * polys), dim3(BC_TILE * BC_WAVES), 0, st, x, in_stride, y,
                            out_stride, t.mod, t.mu, t.r_hatinv, t.r_hat_mod_q, t.alphaR_mod_q, t.r_inv,

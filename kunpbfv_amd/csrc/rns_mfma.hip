@@ -236,14 +236,15 @@ int launch_rns_gemm(int kc, int nj, dim3 grid, hipStream_t st, const uint64_t *x
 bool kb_rns_gemm_fits(int k_rows, int out_n) { return k_rows <= 32 && out_n <= 32; }
 
 int kb_bconv_mfma(const kbfv_context *c, bool q_to_r, const uint64_t *x, long in_stride, uint64_t *y,
-                  long out_stride, int polys, hipStream_t st) {
+                  long out_stride, int polys, hipStream_t st, bool noscale) {
     const int in_n = q_to_r ? c->qn : c->rn, out_n = q_to_r ? c->rn : c->qn;
     const int kc = (in_n + 1 + 7) / 8, nj = (out_n + 15) / 16;
     const kb_dev_tables &t = c->t;
     const int in0 = q_to_r ? c->oQ : c->oR, out0 = q_to_r ? c->oR : c->oQ;
     const dim3 grid((unsigned)((long)polys * (c->n / RG_BLOCK)));
     if (launch_rns_gemm<false>(kc, nj, grid, st, x, in_stride, y, out_stride, q_to_r ? t.mx_qr : t.mx_rq,
-                               t.mod + out0, q_to_r ? t.q_hatinv : t.r_hatinv, t.mod + in0,
+                               t.mod + out0, q_to_r ? (noscale ? t.q_hatinv_n : t.q_hatinv) : t.r_hatinv,
+                               t.mod + in0,
                                q_to_r ? t.q_inv : t.r_inv, in_n, out_n, c->n))
         return 1;
     KB_LAUNCH_CHECK();

@@ -281,12 +281,13 @@ int mult_chunk(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1, 
                uint64_t *out, int cb, const MultWs &w, hipStream_t st) {
     const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn;
     const long CTS = 2 * qn * N;  // ciphertext stride
-    // 1. INTT of both inputs (out of place; the NTT-domain inputs stay for the tensor)
-    KB_TRY(kb_ntt(c, -1, ct1, w.X1, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
-    KB_TRY(kb_ntt(c, -1, ct2, w.X2, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
+    // 1. INTT of both inputs (out of place; the NTT-domain inputs stay for the
+    // tensor), left N times too large: the base extension folds n^-1 in
+    KB_TRY(kb_intt_noscale(c, ct1, w.X1, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
+    KB_TRY(kb_intt_noscale(c, ct2, w.X2, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st));
     // 2. HPS base extension Q -> R, 3. NTT over R
     // (X2 follows X1 and R2 follows R1: one launch covers both ciphertexts)
-    KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * cb, st));
+    KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * cb, st, true));
     KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * cb, 1, 0, rn * N, N, 0, rn * N, N, st));
     // 4. tensor over QR, 5. INTT over QR
     KB_TRY(kb_tensor_qr(c, ct1, ct2, w.R1, w.R2, w.D3, cb, st));
@@ -296,12 +297,10 @@ int mult_chunk(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1, 
     KB_TRY(kb_bconv(c, false, w.SR, rn * N, w.DQ, qn * N, 3 * cb, st));
     // 8. relinearise a1a2: digits -> NTT over T
     KB_TRY(kb_digits_to_t_ntt(c, w.DQ, 3 * qn * N, w.CT, cb, nullptr, st));
-    // 9. key-stationary inner product, 10. INTT over T
+    // 9. key-stationary inner product; 10. INTT over T, 11. T -> PQ and
+    // 12. modDown with the a1b2+a2b1 and b1b2 terms added
     KB_TRY(kb_inner_product(c, w.CT, rlk, w.IP, cb, st));
-    KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * cb, 1, 0, (long)c->tn * N, N, 0,
-                  (long)c->tn * N, N, st));
-    // 11. T -> PQ and 12. modDown with the a1b2+a2b1 and b1b2 terms added, in one kernel
-    KB_TRY(kb_modup_moddown(c, w.IP, w.MD, w.DQ + qn * N, 3 * qn * N, cb, w.PQ, st));
+    KB_TRY(kb_ks_tail(c, w.IP, w.IP, w.MD, w.DQ + qn * N, 3 * qn * N, cb, w.PQ, st));
     // 13. back to the NTT domain, into the output ciphertexts
     return kb_ntt(c, +1, w.MD, out, c->oQ, qn, 2, cb, CTS, qn * N, N, CTS, qn * N, N, st);
 }
@@ -542,9 +541,9 @@ extern "C" int kbfv_mult_before_ip(const kbfv_context *c, const uint64_t *ct1, c
         mult_layout(c, n, ws, &w);
         const uint64_t *a = ct1 + b0 * CTS, *b = ct2 + b0 * CTS;
         // steps 1-8 of the fused pipeline (mult_chunk), the digits going to ip_in
-        KB_TRY(kb_ntt(c, -1, a, w.X1, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
-        KB_TRY(kb_ntt(c, -1, b, w.X2, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
-        KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * n, s));
+        KB_TRY(kb_intt_noscale(c, a, w.X1, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_intt_noscale(c, b, w.X2, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * n, s, true));
         KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * n, 1, 0, rn * N, N, 0, rn * N, N, s));
         KB_TRY(kb_tensor_qr(c, a, b, w.R1, w.R2, w.D3, n, s));
         KB_TRY(kb_ntt(c, -1, w.D3, w.D3, c->oQ, sn, 3 * n, 1, 0, sn * N, N, 0, sn * N, N, s));
@@ -572,9 +571,7 @@ extern "C" int kbfv_mult_after_ip(const kbfv_context *c, const uint64_t *ip_out,
     for (int b0 = 0; b0 < batch; b0 += cb) {
         const int n = std::min(cb, batch - b0);
         // steps 10-13 of the fused pipeline; the caller's inner-product output is left as it is
-        KB_TRY(kb_ntt(c, -1, ip_out + b0 * IPO, w.IP, c->oT, c->tn, 2 * c->ri_blocks * n, 1, 0,
-                      (long)c->tn * N, N, 0, (long)c->tn * N, N, s));
-        KB_TRY(kb_modup_moddown(c, w.IP, w.MD, d01 + b0 * CTS, CTS, n, w.PQ, s));
+        KB_TRY(kb_ks_tail(c, ip_out + b0 * IPO, w.IP, w.MD, d01 + b0 * CTS, CTS, n, w.PQ, s));
         KB_TRY(kb_ntt(c, +1, w.MD, out + b0 * CTS, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
     }
     return KBFV_OK;
@@ -695,9 +692,7 @@ extern "C" int kbfv_rotate(const kbfv_context *c, const uint64_t *gkey, uint64_t
         KB_TRY(kb_ntt(c, -1, ct + b0 * CTS, w.X, c->oQ, qn, 2, nb, CTS, qn * N, N, CTS, qn * N, N, s));
         KB_TRY(kb_digits_to_t_ntt(c, w.X, CTS, w.CT, nb, nullptr, s));
         KB_TRY(kb_inner_product(c, w.CT, gkey, w.IP, nb, s));
-        KB_TRY(kb_ntt(c, -1, w.IP, w.IP, c->oT, c->tn, 2 * c->ri_blocks * nb, 1, 0, (long)c->tn * N, N, 0,
-                      (long)c->tn * N, N, s));
-        KB_TRY(kb_modup_moddown(c, w.IP, w.MD, nullptr, 0, nb, w.PQ, s));
+        KB_TRY(kb_ks_tail(c, w.IP, w.IP, w.MD, nullptr, 0, nb, w.PQ, s));
         for (int b = 0; b < nb; ++b)  // KS(a) + (0, b): b joins the second polynomial only
             KB_TRY(kb_poly_binop(c, 0, w.MD + b * CTS + qn * N, w.X + b * CTS + qn * N, w.MD + b * CTS + qn * N,
                                  c->oQ, qn, 1, s));
