@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the mixed-N serving measurement")
     ap.add_argument("--serving-qps", type=float, default=3000.0)
     ap.add_argument("--serving-seconds", type=float, default=3.0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in NTT plans")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernel and a 4 GiB copy (for counter runs)")
     return ap.parse_args()
@@ -113,6 +114,10 @@ def main():
     ctx = kb.Context.chain60(args.log_n, args.qn, 1, 4, 2, 65537)
     B, N = args.batch, ctx.n
     stream = torch.cuda.current_stream()
+    tuned = None
+    if not args.no_autotune:
+        tuned = autotune_report(ctx, ctx.autotune(B))
+        torch.cuda.empty_cache()
     if args.roofline_only:
         # calibration: a plain device copy of 4 GiB (4 GiB read + 4 GiB written)
         src = torch.empty(1 << 29, dtype=torch.int64, device="cuda")
@@ -222,6 +227,7 @@ def main():
         "step_latency_ms": {"p50": round(pct(step_ms, 50), 3), "p99": round(pct(step_ms, 99), 3)},
         "wall_s": round(wall, 3),
         "roofline": roof,
+        "ntt_autotune": tuned,
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, ct1, ct2, tuple(rlk.shape))
@@ -241,6 +247,7 @@ def graph_c3(stream, batch=64, reps=5):
     import torch
     import kunpbfv_amd as kb
     ctx = kb.Context.chain60(15, 13, 1, 4, 2, 65537)
+    ctx.autotune(batch)
     sk = ctx.keygen_sk(bytes([7]) * 32)
     pk = ctx.keygen_pk(bytes([8]) * 32, sk)
     rlk = ctx.keygen_rlk(bytes([9]) * 32, sk)
@@ -308,6 +315,18 @@ def serving_mix(qps, seconds, seed=11):
                         f"{seed}", "batched": batched, "serial_plain": plain}
 
 
+def autotune_report(ctx, tuned):
+    """The NTT plans the autotuner picked, per direction and transform-count bucket."""
+    import kunpbfv_amd as kb
+    rows = []
+    for (direction, bucket), (plan, ms, transforms) in sorted(tuned.items()):
+        cs, ct, rt, cx, rx = kb.ntt_plan_fields(plan, ctx.log_n)
+        rows.append({"dir": "forward" if direction > 0 else "inverse", "bucket": bucket,
+                     "transforms": transforms, "plan": hex(plan), "split_log": [cs, ctx.log_n - cs],
+                     "tile_log": [ct, rt], "xcd_order": [cx, rx], "ms": round(ms, 4)})
+    return rows
+
+
 def kernel_roofline(ctx, B, stream):
     """Average duration of one launch of the column-pass INTT kernel over the QR
     basis for the whole batch, measured with HIP events on the stream the
@@ -336,7 +355,8 @@ def kernel_roofline(ctx, B, stream):
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": NTT_TRAFFIC_BYTES.get((ctx.log_n, ctx.qn, B)),
-            "launch_ms": round(per_launch_s * 1e3, 3), "bytes_per_launch": bytes_per_launch}
+            "launch_ms": round(per_launch_s * 1e3, 3), "bytes_per_launch": bytes_per_launch,
+            "ntt_plan": hex(ctx.ntt_plan(-1, kb.lib.kbfv_ntt_plan_bucket(polys * sn)))}
 
 
 if __name__ == "__main__":

@@ -89,6 +89,28 @@ int kbfv_ntt(const kbfv_context *ctx, int dir, const uint64_t *src, uint64_t *ds
              int limbs, int polys, int batch, int64_t s_batch, int64_t s_poly, int64_t s_limb,
              int64_t d_batch, int64_t d_poly, int64_t d_limb, void *stream);
 
+/* NTT plans and the autotuner (no counterpart in the reference, whose NTT has
+ * one fixed radix-8 shape, include/ntt_60bit.cuh:414-823).  A transform runs as
+ * a column pass of 2^c-point sub-transforms and a row pass of 2^(logN-c)-point
+ * ones.  Plan word: bits 0-3 = c, bits 4-7 / 8-11 = log2 of the column / row
+ * pass tile (elements per workgroup, 11..13), bit 12 / 13 = XCD-ordered grid
+ * for the column / row pass; 0 = the built-in plan.  Every plan computes the
+ * same bits.  A context keeps one plan per direction and bucket of
+ * limbs*polys*batch (<64, <512, <4096, more); kbfv_ntt uses it. */
+#define KBFV_NTT_BUCKETS 4
+int kbfv_ntt_plan_valid(int log_n, uint32_t plan);                  /* 0 if valid; no device */
+int kbfv_ntt_plan_candidates(int log_n, uint32_t *out, int cap);    /* returns the count */
+int kbfv_ntt_plan_bucket(int64_t transforms);                       /* bucket of a transform count */
+int kbfv_ntt_plan_get(const kbfv_context *ctx, int dir, int bucket, uint32_t *plan);
+int kbfv_ntt_plan_set(kbfv_context *ctx, int dir, int bucket, uint32_t plan);
+/* Times every pass variant (split, tile, grid order) of direction dir over
+ * `polys` contiguous [polys][limbs][N] polynomials in `scratch` (overwritten,
+ * polys*limbs*N words) on `stream`, `reps` launches each, and stores the
+ * fastest plan for the bucket of limbs*polys.  best/best_ms may be null.  Run
+ * it before other work on the context: it writes the plan table other calls read. */
+int kbfv_ntt_autotune(kbfv_context *ctx, int dir, int mod0, int limbs, int polys, uint64_t *scratch,
+                      int reps, void *stream, uint32_t *best, float *best_ms);
+
 /* ---------------------------------------------------------------- RNS kernels */
 
 /* element-wise over `polys` polynomials [polys][limbs][N] (replaces

@@ -35,6 +35,13 @@ SIGNATURES = {
     "kbfv_context_export_f64": (C.c_int, [_vp, C.c_char_p, _f64p, C.c_int]),
     "kbfv_ntt": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _i64, _i64, _i64,
                            _i64, _i64, _i64, _vp]),
+    "kbfv_ntt_plan_valid": (C.c_int, [C.c_int, C.c_uint32]),
+    "kbfv_ntt_plan_candidates": (C.c_int, [C.c_int, C.POINTER(C.c_uint32), C.c_int]),
+    "kbfv_ntt_plan_bucket": (C.c_int, [_i64]),
+    "kbfv_ntt_plan_get": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
+    "kbfv_ntt_plan_set": (C.c_int, [_vp, C.c_int, C.c_int, C.c_uint32]),
+    "kbfv_ntt_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp,
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "kbfv_poly_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_poly_sub": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_poly_mul": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
@@ -104,6 +111,12 @@ class KbfvError(RuntimeError):
 def _load(path=LIB_PATH):
     if not os.path.exists(path):
         raise KbfvError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    # torch ships its own libamdhip64.so with the soname libamdhip64.so.7 that
+    # the library links against.  Loading torch first makes the library bind to
+    # that one HIP runtime; loaded the other way round, the process holds two
+    # runtimes, torch's finds no device, and torch's streams are foreign
+    # handles to ours.
+    import torch  # noqa: F401
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -124,6 +137,23 @@ def check(rc):
 def _arr(vals):
     a = np.ascontiguousarray(np.asarray(vals, dtype=np.uint64))
     return a, a.ctypes.data_as(_u64p)
+
+
+def ntt_plan_candidates(log_n):
+    """Every valid NTT plan word at N = 2^log_n (host only)."""
+    n = lib.kbfv_ntt_plan_candidates(log_n, None, 0)
+    check(min(n, 0))
+    out = (C.c_uint32 * n)()
+    lib.kbfv_ntt_plan_candidates(log_n, out, n)
+    return list(out)
+
+
+def ntt_plan_fields(plan, log_n):
+    """Decode a plan word: (column-pass log length, column tile log, row tile log,
+    column XCD order, row XCD order); 0 is the built-in plan."""
+    if plan == 0:
+        return ((log_n + 1) // 2, 12, 12, 1, 1)
+    return (plan & 15, (plan >> 4) & 15, (plan >> 8) & 15, (plan >> 12) & 1, (plan >> 13) & 1)
 
 
 def chain_60bit(qn, pn, tn):
@@ -281,6 +311,54 @@ class Context:
         check(lib.kbfv_ntt(self.h, direction, ptr(src), ptr(dst), mod0, limbs, polys, batch, *s, *d,
                            stream_ptr(stream)))
         return dst
+
+    # ------------------------------------------------------------ NTT plans / autotuner
+    def ntt_plan(self, direction, bucket):
+        v = C.c_uint32()
+        check(lib.kbfv_ntt_plan_get(self.h, direction, bucket, C.byref(v)))
+        return v.value
+
+    def set_ntt_plan(self, direction, bucket, plan):
+        check(lib.kbfv_ntt_plan_set(self.h, direction, bucket, plan))
+
+    def autotune_ntt(self, direction, mod0, limbs, polys, reps=3, scratch=None, stream=None):
+        """Time every NTT plan for `polys` polynomials of `limbs` limbs from
+        modulus mod0 and keep the fastest for that transform-count bucket.
+        Returns (plan, ms per transform, bucket)."""
+        torch = _torch()
+        words = polys * limbs * self.n
+        if scratch is None:
+            scratch = torch.empty(words, dtype=torch.int64, device="cuda")
+        assert scratch.numel() * scratch.element_size() >= words * 8, "scratch too small"
+        best, ms = C.c_uint32(), C.c_float()
+        check(lib.kbfv_ntt_autotune(self.h, direction, mod0, limbs, polys, ptr(scratch), reps,
+                                    stream_ptr(stream), C.byref(best), C.byref(ms)))
+        return best.value, ms.value, lib.kbfv_ntt_plan_bucket(limbs * polys)
+
+    def autotune(self, batch, scratch=None, reps=3, stream=None):
+        """Tune the NTT plan of every direction and transform-count bucket at
+        the sizes HomMult+relin of `batch` ciphertexts launches: over the QR
+        basis, 1 polynomial (< 64 transforms), 3 (the per-ciphertext tensor),
+        a mid-size batch and the whole batch's 3 tensor polynomials.  Returns
+        {(direction, bucket): (plan, ms per transform, transforms)}."""
+        torch = _torch()
+        limbs = self.qn + self.rn
+        sizes = {}
+        for polys in (1, 3, max(1, 2048 // limbs), 3 * max(1, batch)):
+            b = lib.kbfv_ntt_plan_bucket(limbs * polys)
+            sizes[b] = max(sizes.get(b, 0), polys)
+        need = max(sizes.values()) * limbs * self.n
+        if scratch is None:
+            scratch = torch.empty(need, dtype=torch.int64, device="cuda")
+        if scratch.numel() * scratch.element_size() < need * 8:
+            raise KbfvError("autotune scratch too small")
+        out = {}
+        for direction in (+1, -1):
+            for b, polys in sorted(sizes.items()):
+                plan, ms, got = self.autotune_ntt(direction, self.oQ, limbs, polys, reps, scratch, stream)
+                assert got == b
+                out[(direction, b)] = (plan, ms, limbs * polys)
+        return out
 
     def poly_op(self, op, a, b, out, mod0, limbs, polys=1, stream=None):
         fn = {"add": lib.kbfv_poly_add, "sub": lib.kbfv_poly_sub, "mul": lib.kbfv_poly_mul}[op]

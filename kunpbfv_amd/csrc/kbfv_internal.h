@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -27,6 +28,11 @@ void kb_set_error(const char *fmt, ...);
             kb_set_error(__VA_ARGS__);  \
             return KBFV_EINVAL;         \
         }                               \
+    } while (0)
+#define KB_TRY(x)            \
+    do {                     \
+        int _rc = (x);       \
+        if (_rc) return _rc; \
     } while (0)
 #define KB_LAUNCH_CHECK()                                                                    \
     do {                                                                                     \
@@ -111,6 +117,10 @@ struct kbfv_context {
     void *d_blob = nullptr;
     kbfv_tw *d_tw = nullptr;
     kb_dev_tables t{};
+
+    // NTT plan per direction (forward, inverse) and transform-count bucket;
+    // 0 = the built-in plan.  Set by kbfv_ntt_plan_set / kbfv_ntt_autotune.
+    std::atomic<uint32_t> ntt_plan[2][KBFV_NTT_BUCKETS] = {};
 };
 
 // per-limb modulus helpers

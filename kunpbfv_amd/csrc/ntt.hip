@@ -20,18 +20,38 @@
 // result passes through an SGPR.  Twiddles are (w, Shoup w') pairs read as one 16-byte load,
 // one load per distinct twiddle per thread.  The grid is ordered limb-slowest so
 // that every workgroup in flight shares one modulus' tables in L2.
+//
+// Plans: the split point (column pass length 2^LOG_S, the row pass takes the
+// rest of log N), the tile of each pass (2^11..2^13 elements per workgroup,
+// 16 values per thread) and the XCD-ordered grid of each pass are template /
+// launch parameters; every plan computes the same bits.  Passes of length 2^9
+// run a radix-32 round and a radix-16 round.  A context keeps one plan per
+// (direction, transform-count bucket); kbfv_ntt_autotune times the candidates
+// on the caller's stream and stores the fastest.
+#include <algorithm>
+
 #include "kbfv_internal.h"
 
 namespace {
 
-constexpr int LOG_R = 4;  // radix-16 register rounds
-constexpr int TILE = 4096;  // elements per workgroup
+constexpr int LOG_R = 4;  // 16 values per thread
 
-template <int LOG_S, bool COLUMN>
+// a pass of 2^log_s-point sub-transforms over tiles of 2^log_t elements: the
+// kernels instantiated for it and the host's plan validation agree on this
+__host__ __device__ constexpr int round0_log(int log_s) { return log_s <= 8 ? (log_s < 4 ? log_s : 4) : 5; }
+__host__ __device__ constexpr bool pass_ok(int log_n, int log_s, int log_t, bool column) {
+    return log_s >= 5 && log_s <= 9 && log_t >= 11 && log_t <= 13 && log_t >= log_s && log_t <= log_n &&
+           (!column || log_t - log_s >= 3) &&  // column rows of >= 64 bytes
+           log_s - round0_log(log_s) >= 1 && log_s - round0_log(log_s) <= LOG_R;
+}
+__host__ __device__ constexpr int col_log_default(int log_n) { return (log_n + 1) / 2; }
+
+template <int LOG_S, bool COLUMN, int LOG_T>
 struct Geo {
+    static constexpr int TILE = 1 << LOG_T;    // elements per workgroup
     static constexpr int S = 1 << LOG_S;
     static constexpr int C = TILE / S;         // sub-transforms per workgroup
-    static constexpr int T = TILE >> LOG_R;    // threads per workgroup (256)
+    static constexpr int T = TILE >> LOG_R;    // threads per workgroup
     static constexpr int ROWP = S + S / 16;    // row pass: one pad word per 16
     static constexpr int WORDS = COLUMN ? TILE : C * ROWP;
     __device__ static __forceinline__ int addr(int c, int k) {
@@ -99,11 +119,11 @@ __device__ __forceinline__ void gs_bfly(uint64_t &a, uint64_t &b, kbfv_tw w, uin
 // The twiddles of the one butterfly group thread threadIdx.x takes in a round
 // (rounds with exactly one group per thread), in the order the round consumes
 // them: level j of the round, sub-block m -> slot (1 << j) - 1 + m.
-template <int LR, int LOG_S, bool COLUMN>
+template <int LR, int LOG_S, bool COLUMN, int LOG_T>
 __device__ __forceinline__ void round_twiddles(kbfv_tw *tws, const int lam0, const kbfv_tw *tab,
                                                const int log_n, const int g0, const int grp0,
                                                const int stride) {
-    using G = Geo<LOG_S, COLUMN>;
+    using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, RR = 1 << LR;
     const int grp = threadIdx.x;
     int c, u;
@@ -136,15 +156,15 @@ __device__ __forceinline__ void round_twiddles(kbfv_tw *tws, const int lam0, con
 // (the rounds whose lanes cover adjacent columns / 16 adjacent elements, i.e.
 // 128-byte runs); it writes them back to the tile or, when GOUT, straight to
 // global memory with the final reduction of the last pass (FIN).
-template <int LR, int LOG_S, bool COLUMN, bool FWD, bool NINV, bool GIN = false, bool GOUT = false,
-          bool FIN = false, bool PRE = false, bool W1 = false>
+template <int LR, int LOG_S, bool COLUMN, int LOG_T, bool FWD, bool NINV, bool GIN = false,
+          bool GOUT = false, bool FIN = false, bool PRE = false, bool W1 = false>
 __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const kbfv_tw *tab,
                                           const int log_n, const int g0, const int grp0,
                                           const int stride, const uint64_t q, const kbfv_tw ninv,
                                           const uint64_t *__restrict__ gin = nullptr,
                                           uint64_t *__restrict__ gout = nullptr,
                                           const kbfv_tw (*pre)[(1 << LR) - 1] = nullptr) {
-    using G = Geo<LOG_S, COLUMN>;
+    using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, RR = 1 << LR;
     constexpr int NG = C * (S >> LR);
     const uint64_t q4 = q << 2;
@@ -236,13 +256,13 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
 }
 
 // coalesced global -> LDS copy of the workgroup's tile
-template <int LOG_S, bool COLUMN>
+template <int LOG_S, bool COLUMN, int LOG_T>
 __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restrict__ in, int grp0,
                                          int stride) {
-    using G = Geo<LOG_S, COLUMN>;
+    using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, T = G::T;
 #pragma unroll
-    for (int j = 0; j < TILE / T; ++j) {
+    for (int j = 0; j < G::TILE / T; ++j) {
         const int e = threadIdx.x + j * T;
         int c, k;
         if constexpr (COLUMN) {
@@ -257,22 +277,17 @@ __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restr
     }
 }
 
-// pass sizes: the column pass takes ceil(log_n / 2) levels, the row pass the rest
-template <int LOG_N, bool COLUMN>
-constexpr int pass_log() { return COLUMN ? (LOG_N + 1) / 2 : LOG_N - (LOG_N + 1) / 2; }
-
-template <int LOG_N, bool COLUMN, bool FWD, bool NOSCALE = false>
-__global__ void __launch_bounds__(TILE >> LOG_R)
+template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE = false>
+__global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
     k_ntt_pass(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
                const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
                const kbfv_tw *__restrict__ w1_tab, const uint64_t *__restrict__ mods, int mod0, int limbs,
-               int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL) {
-    constexpr int LOG_S = pass_log<LOG_N, COLUMN>();
+               int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL, int xcd) {
     constexpr int log_n = LOG_N;
-    using G = Geo<LOG_S, COLUMN>;
+    using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, T = G::T;
     constexpr int groups = (1 << (LOG_N - LOG_S)) / C;  // workgroups per limb
-    static_assert(groups >= 1, "ring too small for the tile");
+    static_assert(pass_ok(LOG_N, LOG_S, LOG_T, COLUMN), "pass shape outside the plan space");
     constexpr bool LAST = (FWD != COLUMN);  // forward ends with the row pass
     __shared__ uint64_t tile[G::WORDS];
 
@@ -280,7 +295,7 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     // multiple of 8 each XCD gets a contiguous range of tiles (the column
     // tiles that share DRAM rows, and one modulus' twiddles, stay in one L2)
     int bid = (int)blockIdx.x;
-    if ((gridDim.x & 7) == 0) bid = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    if (xcd && (gridDim.x & 7) == 0) bid = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
     const int grp0 = bid % groups;
     bid /= groups;
     const int y = bid % polys;
@@ -297,7 +312,7 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     constexpr int stride = n >> LOG_S;
     constexpr int g0 = COLUMN ? 0 : (log_n - LOG_S);
 
-    constexpr int LR0 = LOG_S < LOG_R ? LOG_S : LOG_R;
+    constexpr int LR0 = round0_log(LOG_S);
     constexpr int LR1 = LOG_S - LR0;
     static_assert(LR1 > 0 && LR1 <= LOG_R, "two register rounds per pass");
     // NOSCALE (inverse column pass): N x INTT, lazy, stage 0 with the
@@ -308,46 +323,47 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     // touch 128-byte runs in both layouts; the stride-1 rounds of the row pass
     // go through the LDS tile for both their global load and store.
     if constexpr (FWD) {
-        ntt_round<LR0, LOG_S, COLUMN, true, false, true>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv, in);
+        ntt_round<LR0, LOG_S, COLUMN, LOG_T, true, false, true>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv,
+                                                                in);
         if constexpr (COLUMN) {
-            ntt_round<LR1, LOG_S, COLUMN, true, false, false, true, LAST>(tile, LR0, tab, log_n, g0, grp0,
-                                                                          stride, q, ninv, nullptr, out);
+            ntt_round<LR1, LOG_S, COLUMN, LOG_T, true, false, false, true, LAST>(tile, LR0, tab, log_n, g0, grp0,
+                                                                                 stride, q, ninv, nullptr, out);
             return;
         } else {
-            ntt_round<LR1, LOG_S, COLUMN, true, false>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
+            ntt_round<LR1, LOG_S, COLUMN, LOG_T, true, false>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
         }
     } else {
         if constexpr (COLUMN) {
-            ntt_round<LR1, LOG_S, COLUMN, false, NI, true>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv,
-                                                           in);
+            ntt_round<LR1, LOG_S, COLUMN, LOG_T, false, NI, true>(tile, LR0, tab, log_n, g0, grp0, stride, q,
+                                                                  ninv, in);
         } else {
             // one butterfly group per thread (a radix-16 round): its twiddles are
             // loaded with the tile, ahead of the barrier, rather than after it
             constexpr bool PRE = (C * (S >> LR1) == T);
             if constexpr (PRE) {
                 kbfv_tw pre[(1 << LR1) - 1];
-                round_twiddles<LR1, LOG_S, COLUMN>(pre, LR0, tab, log_n, g0, grp0, stride);
-                stage_in<LOG_S, COLUMN>(tile, in, grp0, stride);
+                round_twiddles<LR1, LOG_S, COLUMN, LOG_T>(pre, LR0, tab, log_n, g0, grp0, stride);
+                stage_in<LOG_S, COLUMN, LOG_T>(tile, in, grp0, stride);
                 // pinned here so that the loads are not sunk below the barrier
 #pragma unroll
                 for (int i = 0; i < (1 << LR1) - 1; ++i) asm volatile("" : "+v"(pre[i].w), "+v"(pre[i].ws));
                 __syncthreads();
-                ntt_round<LR1, LOG_S, COLUMN, false, NI, false, false, false, true>(
+                ntt_round<LR1, LOG_S, COLUMN, LOG_T, false, NI, false, false, false, true>(
                     tile, LR0, tab, log_n, g0, grp0, stride, q, ninv, nullptr, nullptr, &pre);
             } else {
-                stage_in<LOG_S, COLUMN>(tile, in, grp0, stride);
+                stage_in<LOG_S, COLUMN, LOG_T>(tile, in, grp0, stride);
                 __syncthreads();
-                ntt_round<LR1, LOG_S, COLUMN, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
+                ntt_round<LR1, LOG_S, COLUMN, LOG_T, false, NI>(tile, LR0, tab, log_n, g0, grp0, stride, q, ninv);
             }
         }
-        ntt_round<LR0, LOG_S, COLUMN, false, NI, false, true, LAST && !NOSCALE, false, NOSCALE>(
+        ntt_round<LR0, LOG_S, COLUMN, LOG_T, false, NI, false, true, LAST && !NOSCALE, false, NOSCALE>(
             tile, 0, tab, log_n, g0, grp0, stride, q, ninv, nullptr, out);
         return;
     }
 
     // forward row pass: stride-1 round done in the tile, coalesced store from it
 #pragma unroll
-    for (int j = 0; j < TILE / T; ++j) {
+    for (int j = 0; j < G::TILE / T; ++j) {
         const int e = threadIdx.x + j * T;
         const int c = e / S, k = e % S;
         uint64_t v = tile[G::addr(c, k)];
@@ -360,39 +376,121 @@ __global__ void __launch_bounds__(TILE >> LOG_R)
     }
 }
 
-template <int LOG_N, bool COLUMN, bool FWD, bool NOSCALE = false>
-int launch_pass(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
-                int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
-                hipStream_t st) {
-    using G = Geo<pass_log<LOG_N, COLUMN>(), COLUMN>;
-    const long groups = (1L << (LOG_N - pass_log<LOG_N, COLUMN>())) / G::C;
-    const long blocks = groups * limbs * polys * batch;
-    KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
-    hipLaunchKernelGGL((k_ntt_pass<LOG_N, COLUMN, FWD, NOSCALE>), dim3((unsigned)blocks), dim3(G::T), 0, st, src, dst,
-                       c->t.tw, c->t.ninv, c->t.inv_w1, c->t.mod, mod0, limbs, polys, batch, sB, sY, sL, dB, dY,
-                       dL);
-    KB_LAUNCH_CHECK();
-    return KBFV_OK;
+struct pass_args {
+    const kbfv_context *c;
+    const uint64_t *src;
+    uint64_t *dst;
+    int mod0, limbs, polys, batch;
+    long sB, sY, sL, dB, dY, dL;
+    int xcd;
+    hipStream_t st;
+};
+
+template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE>
+int launch_pass(const pass_args &a) {
+    if constexpr (!pass_ok(LOG_N, LOG_S, LOG_T, COLUMN)) {
+        kb_set_error("NTT pass 2^%d over tiles of 2^%d is not in the plan space", LOG_S, LOG_T);
+        return KBFV_EINVAL;
+    } else {
+        using G = Geo<LOG_S, COLUMN, LOG_T>;
+        const long groups = (1L << (LOG_N - LOG_S)) / G::C;
+        const long blocks = groups * a.limbs * a.polys * a.batch;
+        KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
+        hipLaunchKernelGGL((k_ntt_pass<LOG_N, LOG_S, LOG_T, COLUMN, FWD, NOSCALE>), dim3((unsigned)blocks), dim3(G::T),
+                           0, a.st, a.src, a.dst, a.c->t.tw, a.c->t.ninv, a.c->t.inv_w1, a.c->t.mod, a.mod0, a.limbs,
+                           a.polys, a.batch, a.sB, a.sY, a.sL, a.dB, a.dY, a.dL, a.xcd);
+        KB_LAUNCH_CHECK();
+        return KBFV_OK;
+    }
+}
+
+// pass length 2^log_s (one of three around the default split) and tile 2^log_t
+template <int LOG_N, bool COLUMN, bool FWD, bool NOSCALE>
+int launch_pass_st(int log_s, int log_t, const pass_args &a) {
+    constexpr int s0 = (COLUMN ? col_log_default(LOG_N) : LOG_N - col_log_default(LOG_N)) - 1;
+#define KB_NTT_ST(DS, LT)                                                   \
+    if (log_s == s0 + DS && log_t == LT) return launch_pass<LOG_N, s0 + DS, LT, COLUMN, FWD, NOSCALE>(a);
+    KB_NTT_ST(0, 11) KB_NTT_ST(0, 12) KB_NTT_ST(0, 13)
+    KB_NTT_ST(1, 11) KB_NTT_ST(1, 12) KB_NTT_ST(1, 13)
+    KB_NTT_ST(2, 11) KB_NTT_ST(2, 12) KB_NTT_ST(2, 13)
+#undef KB_NTT_ST
+    kb_set_error("NTT pass 2^%d over tiles of 2^%d is not in the plan space", log_s, log_t);
+    return KBFV_EINVAL;
 }
 
 template <bool COLUMN, bool FWD, bool NOSCALE = false>
-int launch_pass_n(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs, int polys,
-                  int batch, long sB, long sY, long sL, long dB, long dY, long dL, hipStream_t st) {
-#define KB_NTT_N(LN)                                                                                     \
-    case LN:                                                                                             \
-        return launch_pass<LN, COLUMN, FWD, NOSCALE>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, \
-                                            dL, st);
-    switch (c->log_n) {
-        KB_NTT_N(12)
-        KB_NTT_N(13)
-        KB_NTT_N(14)
-        KB_NTT_N(15)
-        KB_NTT_N(16)
+int launch_pass_n(int log_s, int log_t, const pass_args &a) {
+    switch (a.c->log_n) {
+        case 12: return launch_pass_st<12, COLUMN, FWD, NOSCALE>(log_s, log_t, a);
+        case 13: return launch_pass_st<13, COLUMN, FWD, NOSCALE>(log_s, log_t, a);
+        case 14: return launch_pass_st<14, COLUMN, FWD, NOSCALE>(log_s, log_t, a);
+        case 15: return launch_pass_st<15, COLUMN, FWD, NOSCALE>(log_s, log_t, a);
+        case 16: return launch_pass_st<16, COLUMN, FWD, NOSCALE>(log_s, log_t, a);
         default:
-            kb_set_error("unsupported ring degree 2^%d", c->log_n);
+            kb_set_error("unsupported ring degree 2^%d", a.c->log_n);
             return KBFV_EINVAL;
     }
-#undef KB_NTT_N
+}
+
+// ---------------------------------------------------------------- plans
+struct plan_fields {
+    int col_s, col_t, row_t, col_x, row_x;
+};
+
+plan_fields decode_plan(int log_n, uint32_t p) {
+    if (p == 0) return {col_log_default(log_n), 12, 12, 1, 1};
+    return {(int)(p & 15), (int)((p >> 4) & 15), (int)((p >> 8) & 15), (int)((p >> 12) & 1), (int)((p >> 13) & 1)};
+}
+
+bool plan_ok(int log_n, uint32_t p) {
+    if (p == 0) return true;
+    if (p >> 14) return false;
+    const plan_fields f = decode_plan(log_n, p);
+    const int d = f.col_s - col_log_default(log_n);
+    return d >= -1 && d <= 1 && pass_ok(log_n, f.col_s, f.col_t, true) &&
+           pass_ok(log_n, log_n - f.col_s, f.row_t, false);
+}
+
+uint32_t encode_plan(int col_s, int col_t, int row_t, int col_x, int row_x) {
+    return (uint32_t)col_s | (uint32_t)col_t << 4 | (uint32_t)row_t << 8 | (uint32_t)col_x << 12 |
+           (uint32_t)row_x << 13;
+}
+
+int bucket_of(long transforms) {
+    return transforms < 64 ? 0 : transforms < 512 ? 1 : transforms < 4096 ? 2 : 3;
+}
+
+uint32_t plan_for(const kbfv_context *c, int dir, long transforms) {
+    return c->ntt_plan[dir > 0 ? 0 : 1][bucket_of(transforms)].load(std::memory_order_relaxed);
+}
+
+int check_args(const kbfv_context *c, const uint64_t *src, const uint64_t *dst, int mod0, int limbs, int polys,
+               int batch) {
+    KB_REQUIRE(c && src && dst, "null argument");
+    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
+    KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod, "modulus range out of bounds");
+    KB_REQUIRE(polys > 0 && batch > 0, "empty NTT");
+    return KBFV_OK;
+}
+
+// one transform with an explicit plan; NOSCALE: inverse without n^-1 and the final reduction
+int ntt_with_plan(const kbfv_context *c, int dir, bool noscale, uint32_t plan, const uint64_t *src, uint64_t *dst,
+                  int mod0, int limbs, int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
+                  hipStream_t st) {
+    const plan_fields f = decode_plan(c->log_n, plan);
+    const int row_s = c->log_n - f.col_s;
+    pass_args first{c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, dir > 0 ? f.col_x : f.row_x, st};
+    pass_args second{c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, dir > 0 ? f.row_x : f.col_x, st};
+    int rc;
+    if (dir > 0) {
+        rc = launch_pass_n<true, true>(f.col_s, f.col_t, first);
+        if (rc) return rc;
+        return launch_pass_n<false, true>(row_s, f.row_t, second);
+    }
+    rc = launch_pass_n<false, false>(row_s, f.row_t, first);
+    if (rc) return rc;
+    if (noscale) return launch_pass_n<true, false, true>(f.col_s, f.col_t, second);
+    return launch_pass_n<true, false>(f.col_s, f.col_t, second);
 }
 
 }  // namespace
@@ -400,21 +498,10 @@ int launch_pass_n(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int
 int kb_ntt(const kbfv_context *c, int dir, const uint64_t *src, uint64_t *dst, int mod0, int limbs,
            int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
            hipStream_t st) {
-    KB_REQUIRE(c && src && dst, "null argument");
-    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
-    KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod, "modulus range out of bounds");
-    KB_REQUIRE(polys > 0 && batch > 0, "empty NTT");
+    KB_TRY(check_args(c, src, dst, mod0, limbs, polys, batch));
     if (mod0 + limbs - 1 == c->oPlain) KB_REQUIRE(c->batching, "plain modulus has no 2N-th root");
-    int rc;
-    if (dir > 0) {
-        rc = launch_pass_n<true, true>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
-        if (rc) return rc;
-        return launch_pass_n<false, true>(c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, st);
-    }
-    rc = launch_pass_n<false, false>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
-    if (rc) return rc;
-    return launch_pass_n<true, false>(c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY,
-                                      dL, st);
+    const uint32_t plan = plan_for(c, dir, (long)limbs * polys * batch);
+    return ntt_with_plan(c, dir, false, plan, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
 }
 
 int kb_ntt_inplace(const kbfv_context *c, int dir, uint64_t *data, int mod0, int limbs, int polys,
@@ -426,11 +513,127 @@ int kb_ntt_inplace(const kbfv_context *c, int dir, uint64_t *data, int mod0, int
 
 int kb_intt_noscale(const kbfv_context *c, const uint64_t *src, uint64_t *dst, int mod0, int limbs, int polys,
                     int batch, long sB, long sY, long sL, long dB, long dY, long dL, hipStream_t st) {
-    KB_REQUIRE(c && src && dst, "null argument");
-    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
-    KB_REQUIRE(mod0 >= 0 && limbs > 0 && mod0 + limbs <= c->nmod, "modulus range out of bounds");
-    KB_REQUIRE(polys > 0 && batch > 0, "empty NTT");
-    const int rc = launch_pass_n<false, false>(c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
-    if (rc) return rc;
-    return launch_pass_n<true, false, true>(c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, st);
+    KB_TRY(check_args(c, src, dst, mod0, limbs, polys, batch));
+    const uint32_t plan = plan_for(c, -1, (long)limbs * polys * batch);
+    return ntt_with_plan(c, -1, true, plan, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
+}
+
+// ---------------------------------------------------------------- plan ABI and autotuner
+extern "C" int kbfv_ntt_plan_valid(int log_n, uint32_t plan) {
+    KB_REQUIRE(log_n >= 12 && log_n <= 16, "NTT supports 2^12..2^16");
+    return plan_ok(log_n, plan) ? KBFV_OK : KBFV_EINVAL;
+}
+
+extern "C" int kbfv_ntt_plan_candidates(int log_n, uint32_t *out, int cap) {
+    KB_REQUIRE(log_n >= 12 && log_n <= 16, "NTT supports 2^12..2^16");
+    KB_REQUIRE(cap >= 0 && (out || cap == 0), "null output");
+    int k = 0;
+    for (int d = -1; d <= 1; ++d)
+        for (int ct = 11; ct <= 13; ++ct)
+            for (int rt = 11; rt <= 13; ++rt)
+                for (int cx = 0; cx <= 1; ++cx)
+                    for (int rx = 0; rx <= 1; ++rx) {
+                        const uint32_t p = encode_plan(col_log_default(log_n) + d, ct, rt, cx, rx);
+                        if (!plan_ok(log_n, p)) continue;
+                        if (k < cap) out[k] = p;
+                        ++k;
+                    }
+    return k;
+}
+
+extern "C" int kbfv_ntt_plan_bucket(int64_t transforms) {
+    KB_REQUIRE(transforms > 0, "empty NTT");
+    return bucket_of((long)transforms);
+}
+
+extern "C" int kbfv_ntt_plan_get(const kbfv_context *c, int dir, int bucket, uint32_t *plan) {
+    KB_REQUIRE(c && plan, "null argument");
+    KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
+    KB_REQUIRE(bucket >= 0 && bucket < KBFV_NTT_BUCKETS, "bucket out of range");
+    *plan = c->ntt_plan[dir > 0 ? 0 : 1][bucket].load(std::memory_order_relaxed);
+    return KBFV_OK;
+}
+
+extern "C" int kbfv_ntt_plan_set(kbfv_context *c, int dir, int bucket, uint32_t plan) {
+    KB_REQUIRE(c, "null argument");
+    KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
+    KB_REQUIRE(bucket >= 0 && bucket < KBFV_NTT_BUCKETS, "bucket out of range");
+    KB_REQUIRE(plan_ok(c->log_n, plan), "NTT plan 0x%x is not valid at N=2^%d", plan, c->log_n);
+    c->ntt_plan[dir > 0 ? 0 : 1][bucket].store(plan, std::memory_order_relaxed);
+    return KBFV_OK;
+}
+
+namespace {
+// average milliseconds of one pass variant over `reps` launches after one warm-up
+int time_pass(const kbfv_context *c, int dir, bool column, int log_s, int log_t, int xcd, uint64_t *buf, int mod0,
+              int limbs, int polys, int reps, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float *ms) {
+    const long N = c->n;
+    const pass_args a{c, buf, buf, mod0, limbs, polys, 1, 0, (long)limbs * N, N, 0, (long)limbs * N, N, xcd, st};
+    auto run = [&]() -> int {
+        if (column) return dir > 0 ? launch_pass_n<true, true>(log_s, log_t, a) : launch_pass_n<true, false>(log_s, log_t, a);
+        return dir > 0 ? launch_pass_n<false, true>(log_s, log_t, a) : launch_pass_n<false, false>(log_s, log_t, a);
+    };
+    KB_TRY(run());
+    KB_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) KB_TRY(run());
+    KB_HIP(hipEventRecord(e1, st));
+    KB_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    KB_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    return KBFV_OK;
+}
+}  // namespace
+
+extern "C" int kbfv_ntt_autotune(kbfv_context *c, int dir, int mod0, int limbs, int polys, uint64_t *scratch,
+                                 int reps, void *stream, uint32_t *best, float *best_ms) {
+    KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
+    KB_TRY(check_args(c, scratch, scratch, mod0, limbs, polys, 1));
+    KB_REQUIRE(mod0 + limbs - 1 != c->oPlain, "tune over RNS limbs, not the plain modulus");
+    KB_REQUIRE(reps >= 1 && reps <= 100, "reps must be in [1, 100]");
+    hipStream_t st = (hipStream_t)stream;
+    const int log_n = c->log_n;
+    // residues of 0: the passes read and write every element whatever its value
+    KB_HIP(hipMemsetAsync(scratch, 0, (size_t)polys * limbs * c->n * sizeof(uint64_t), st));
+    hipEvent_t e0, e1;
+    KB_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        kb_set_error("hipEventCreate failed");
+        return KBFV_EHIP;
+    }
+    int rc = KBFV_OK;
+    float best_total = 1e30f;
+    uint32_t best_plan = 0;
+    // the two passes are separate launches over the same data: tune each pass's
+    // tile and grid order for every split, then take the split with the least sum
+    for (int d = -1; d <= 1 && rc == KBFV_OK; ++d) {
+        const int cs = col_log_default(log_n) + d, rs = log_n - cs;
+        float bc = 1e30f, br = 1e30f;
+        int bct = 0, bcx = 0, brt = 0, brx = 0;
+        for (int t = 11; t <= 13 && rc == KBFV_OK; ++t)
+            for (int x = 0; x <= 1 && rc == KBFV_OK; ++x) {
+                float ms;
+                if (pass_ok(log_n, cs, t, true)) {
+                    rc = time_pass(c, dir, true, cs, t, x, scratch, mod0, limbs, polys, reps, st, e0, e1, &ms);
+                    if (rc == KBFV_OK && ms < bc) bc = ms, bct = t, bcx = x;
+                }
+                if (rc == KBFV_OK && pass_ok(log_n, rs, t, false)) {
+                    rc = time_pass(c, dir, false, rs, t, x, scratch, mod0, limbs, polys, reps, st, e0, e1, &ms);
+                    if (rc == KBFV_OK && ms < br) br = ms, brt = t, brx = x;
+                }
+            }
+        if (rc == KBFV_OK && bct && brt && bc + br < best_total) {
+            best_total = bc + br;
+            best_plan = encode_plan(cs, bct, brt, bcx, brx);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != KBFV_OK) return rc;
+    KB_REQUIRE(best_plan != 0, "no NTT plan fits N=2^%d", log_n);
+    c->ntt_plan[dir > 0 ? 0 : 1][bucket_of((long)limbs * polys)].store(best_plan, std::memory_order_relaxed);
+    if (best) *best = best_plan;
+    if (best_ms) *best_ms = best_total;
+    return KBFV_OK;
 }

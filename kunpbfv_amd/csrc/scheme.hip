@@ -232,12 +232,6 @@ int fill_zero(uint64_t *p, long total, hipStream_t st) {
     return KBFV_OK;
 }
 
-#define KB_TRY(x)                  \
-    do {                           \
-        int _rc = (x);             \
-        if (_rc) return _rc;       \
-    } while (0)
-
 // ---------------------------------------------------------------- HomMult layout
 struct MultWs {
     uint64_t *X1, *X2;      // coefficient-form inputs [cb][2][qn][N]      (region A)

@@ -53,3 +53,18 @@ def test_errors_are_reported_not_raised_across_the_abi():
     h = ctypes.c_void_p()
     rc = lib.kbfv_context_create(9, q, 1, q, 1, q, 1, 1, 65537, ctypes.byref(h))
     assert rc == -1 and b"logN" in lib.kbfv_last_error()
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """torch bundles a libamdhip64.so with the library's soname: importing the
+    package before torch must still leave exactly one HIP runtime mapped (two
+    runtimes in one process leave torch without a device)."""
+    import subprocess
+    import sys
+    code = ("import kunpbfv_amd, torch\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(libs), sorted(libs)[0])\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, check=True)
+    count, path = out.stdout.split()
+    assert count == "1", out.stdout
+    assert "torch" in path, path

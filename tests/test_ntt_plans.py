@@ -1,0 +1,58 @@
+"""NTT plan space (host only, no device calls): the candidate list holds the
+built-in plan's shape and exactly the (split, tile, grid order) combinations
+whose passes the kernels cover, and the validator rejects malformed words."""
+import pytest
+
+
+def expect_pass_ok(log_n, log_s, log_t, column):
+    # 2^log_s-point sub-transforms, tiles of 2^log_t elements (16 values per
+    # thread): two register rounds (radix-32 only at 2^9), at least one tile
+    # per limb and, for the column pass, rows of at least 8 columns (64 bytes)
+    if not (5 <= log_s <= 9 and 11 <= log_t <= 13):
+        return False
+    if log_t < log_s or log_t > log_n:
+        return False
+    return not (column and log_t - log_s < 3)
+
+
+@pytest.mark.parametrize("log_n", [12, 13, 14, 15, 16])
+def test_candidates_are_exactly_the_valid_shapes(log_n):
+    import kunpbfv_amd as kb
+    got = kb.ntt_plan_candidates(log_n)
+    want = set()
+    c0 = (log_n + 1) // 2
+    for cs in (c0 - 1, c0, c0 + 1):
+        for ct in (11, 12, 13):
+            for rt in (11, 12, 13):
+                if expect_pass_ok(log_n, cs, ct, True) and expect_pass_ok(log_n, log_n - cs, rt, False):
+                    for cx in (0, 1):
+                        for rx in (0, 1):
+                            want.add(cs | ct << 4 | rt << 8 | cx << 12 | rx << 13)
+    assert set(got) == want and len(got) == len(want)
+    for p in got:
+        assert kb.lib.kbfv_ntt_plan_valid(log_n, p) == 0
+        f = kb.ntt_plan_fields(p, log_n)
+        assert f[0] + (log_n - f[0]) == log_n
+    # the built-in plan (0) is the balanced split, 4096-element tiles, XCD order
+    assert kb.lib.kbfv_ntt_plan_valid(log_n, 0) == 0
+    builtin = c0 | 12 << 4 | 12 << 8 | 1 << 12 | 1 << 13
+    assert builtin in got and kb.ntt_plan_fields(0, log_n) == kb.ntt_plan_fields(builtin, log_n)
+
+
+def test_large_rings_offer_radix32_splits():
+    import kunpbfv_amd as kb
+    splits = {p & 15 for p in kb.ntt_plan_candidates(16)}
+    assert splits == {7, 8, 9}
+    # a 2^9-point column pass needs 8 columns of 512: tiles of 2^12 or more
+    assert not any((p & 15) == 9 and ((p >> 4) & 15) == 11 for p in kb.ntt_plan_candidates(16))
+
+
+def test_validator_rejects_malformed_words():
+    import kunpbfv_amd as kb
+    lib = kb.lib
+    assert lib.kbfv_ntt_plan_valid(16, 1 << 14 | 8 | 12 << 4 | 12 << 8) != 0   # unknown bit
+    assert lib.kbfv_ntt_plan_valid(16, 10 | 13 << 4 | 13 << 8) != 0           # split too far off
+    assert lib.kbfv_ntt_plan_valid(12, 6 | 13 << 4 | 12 << 8) != 0            # tile larger than the ring
+    assert lib.kbfv_ntt_plan_valid(11, 0) != 0                                 # ring degree
+    assert [lib.kbfv_ntt_plan_bucket(n) for n in (1, 63, 64, 511, 512, 4095, 4096, 1 << 20)] == \
+        [0, 0, 1, 1, 2, 2, 3, 3]
