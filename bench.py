@@ -25,11 +25,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes per k_ntt_pass launch of the roofline measurement at N=2^16,
-# 28 Q + 29 R primes, 3·128 polynomials: rocprofv3 FETCH_SIZE and WRITE_SIZE
-# runs over `bench.py --roofline-only`, each scaled by the factor the same
-# counter reads for a 4 GiB device copy (fetch 2.000, write 1.000); recorded
-# in profiles/ntt_qr_inverse_n65536_b128_counters.md.
-NTT_TRAFFIC_BYTES = {(16, 28, 128): 22_991_137_126}
+# 28 Q + 29 R primes, 3·128 polynomials (tuned plan: row tile 2^12, column
+# tile 2^13): rocprofv3 FETCH_SIZE and WRITE_SIZE runs over
+# `bench.py --roofline-only`, each scaled by the factor the same counter reads
+# for a 4 GiB device copy (fetch 2.000, write 1.000); recorded in
+# profiles/ntt_qr_inverse_n65536_b128_counters.md.
+NTT_TRAFFIC_BYTES = {(16, 28, 128): 23_000_583_594}
 
 
 def parse():

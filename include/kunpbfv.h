@@ -111,6 +111,22 @@ int kbfv_ntt_plan_set(kbfv_context *ctx, int dir, int bucket, uint32_t plan);
 int kbfv_ntt_autotune(kbfv_context *ctx, int dir, int mod0, int limbs, int polys, uint64_t *scratch,
                       int reps, void *stream, uint32_t *best, float *best_ms);
 
+/* Ragged batches: polynomials of different RNS lengths (ciphertexts at uneven
+ * levels, keys over P+Q next to ciphertexts over Q) transformed in one launch
+ * per pass.  Replaces a host loop of ToNTTInplace / FromNTTInplace calls, one
+ * per polynomial and level (include/BFVcontext.cuh:1513-1552).  Descriptor i is
+ * six int64 words {src, dst, s_limb, d_limb, mod0, limbs}: limb l (modulus
+ * mod0 + l) at src + l*s_limb, written to dst + l*d_limb (src may equal dst;
+ * limbs may be 0).  kbfv_ntt_desc_upload validates `count` host descriptors
+ * against the context and copies them with their limb offsets (count + 1
+ * int32) to caller device buffers, returning the total limb count;
+ * kbfv_ntt_desc then reads only device memory, so it can be captured in a HIP
+ * graph and replayed while the descriptors stay valid.  Built-in plan. */
+int kbfv_ntt_desc_upload(const kbfv_context *ctx, const int64_t *host_desc, int count, int64_t *dev_desc,
+                         int32_t *dev_limb_start, int *total_limbs, void *stream);
+int kbfv_ntt_desc(const kbfv_context *ctx, int dir, const int64_t *dev_desc, const int32_t *dev_limb_start,
+                  int count, int total_limbs, void *stream);
+
 /* ---------------------------------------------------------------- RNS kernels */
 
 /* element-wise over `polys` polynomials [polys][limbs][N] (replaces

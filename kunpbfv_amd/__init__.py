@@ -42,6 +42,8 @@ SIGNATURES = {
     "kbfv_ntt_plan_set": (C.c_int, [_vp, C.c_int, C.c_int, C.c_uint32]),
     "kbfv_ntt_autotune": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp,
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "kbfv_ntt_desc_upload": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _ip, _vp]),
+    "kbfv_ntt_desc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp]),
     "kbfv_poly_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_poly_sub": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_poly_mul": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
@@ -360,6 +362,20 @@ class Context:
                 out[(direction, b)] = (plan, ms, limbs * polys)
         return out
 
+    # ------------------------------------------------------------ ragged batches
+    def ragged(self, items, stream=None):
+        """Upload descriptors of a ragged batch: items are (src, dst, mod0, limbs)
+        with src/dst device tensors holding [limbs][N] (or (src, dst, mod0, limbs,
+        s_limb, d_limb) for other limb strides, in uint64 units)."""
+        return RaggedBatch(self, items, stream)
+
+    def ntt_ragged(self, direction, batch, stream=None):
+        """One batched NTT (forward +1 / inverse -1) over every polynomial of a
+        RaggedBatch, whatever their RNS lengths; reads only device memory, so
+        it may be captured in a graph."""
+        check(lib.kbfv_ntt_desc(self.h, direction, ptr(batch.desc), ptr(batch.start), batch.count, batch.total,
+                                stream_ptr(stream)))
+
     def poly_op(self, op, a, b, out, mod0, limbs, polys=1, stream=None):
         fn = {"add": lib.kbfv_poly_add, "sub": lib.kbfv_poly_sub, "mul": lib.kbfv_poly_mul}[op]
         check(fn(self.h, ptr(a), ptr(b), ptr(out), mod0, limbs, polys, stream_ptr(stream)))
@@ -587,6 +603,29 @@ class Context:
         check(lib.kbfv_external_product(self.h, ptr(rgsw), ptr(ct), ptr(out), batch, base_bits, ptr(ws),
                                         ws.numel(), stream_ptr(stream)))
         return out
+
+
+class RaggedBatch:
+    """Device descriptors of polynomials of different RNS lengths (see
+    kbfv_ntt_desc_upload); keeps the tensors it names alive."""
+
+    def __init__(self, ctx, items, stream=None):
+        torch = _torch()
+        desc = np.zeros((len(items), 6), dtype=np.int64)
+        for i, it in enumerate(items):
+            src, dst, mod0, limbs = it[:4]
+            s_limb, d_limb = (it[4], it[5]) if len(it) > 4 else (ctx.n, ctx.n)
+            for t, st in ((src, s_limb), (dst, d_limb)):
+                if limbs and t.numel() < (limbs - 1) * st + ctx.n:
+                    raise KbfvError(f"descriptor {i}: tensor holds fewer than {limbs} limbs")
+            desc[i] = (src.data_ptr(), dst.data_ptr(), s_limb, d_limb, mod0, limbs)
+        self.desc = torch.empty(desc.size, dtype=torch.int64, device="cuda")
+        self.start = torch.empty(len(items) + 1, dtype=torch.int32, device="cuda")
+        total = C.c_int()
+        check(lib.kbfv_ntt_desc_upload(ctx.h, desc.ctypes.data, len(items), ptr(self.desc), ptr(self.start),
+                                       C.byref(total), stream_ptr(stream)))
+        self.count, self.total = len(items), total.value
+        self._keep = [it[:2] for it in items]
 
 
 class MultGraph:

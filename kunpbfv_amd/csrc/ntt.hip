@@ -277,38 +277,20 @@ __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restr
     }
 }
 
-template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE = false>
-__global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
-    k_ntt_pass(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
-               const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
-               const kbfv_tw *__restrict__ w1_tab, const uint64_t *__restrict__ mods, int mod0, int limbs,
-               int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL, int xcd) {
+// one workgroup's tile of one limb: `in` / `out` address the limb, m is its modulus
+template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE>
+__device__ __forceinline__ void pass_body(uint64_t *tile, const uint64_t *__restrict__ in,
+                                          uint64_t *__restrict__ out, const kbfv_tw *__restrict__ tw,
+                                          const kbfv_tw *__restrict__ ninv_tab, const kbfv_tw *__restrict__ w1_tab,
+                                          const uint64_t *__restrict__ mods, const int m, const int grp0) {
     constexpr int log_n = LOG_N;
     using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, T = G::T;
-    constexpr int groups = (1 << (LOG_N - LOG_S)) / C;  // workgroups per limb
     static_assert(pass_ok(LOG_N, LOG_S, LOG_T, COLUMN), "pass shape outside the plan space");
     constexpr bool LAST = (FWD != COLUMN);  // forward ends with the row pass
-    __shared__ uint64_t tile[G::WORDS];
-
-    // XCD-aware order: workgroup b runs on XCD b % 8, so when the grid is a
-    // multiple of 8 each XCD gets a contiguous range of tiles (the column
-    // tiles that share DRAM rows, and one modulus' twiddles, stay in one L2)
-    int bid = (int)blockIdx.x;
-    if (xcd && (gridDim.x & 7) == 0) bid = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
-    const int grp0 = bid % groups;
-    bid /= groups;
-    const int y = bid % polys;
-    bid /= polys;
-    const int bi = bid % batch;
-    const int l = bid / batch;
-
     constexpr int n = 1 << log_n;
-    const int m = mod0 + l;
     const uint64_t q = mods[m];
     const kbfv_tw *tab = tw + ((size_t)m * 2 + (FWD ? 0 : 1)) * n;
-    const uint64_t *in = src + bi * sB + y * sY + l * sL;
-    uint64_t *out = dst + bi * dB + y * dY + l * dL;
     constexpr int stride = n >> LOG_S;
     constexpr int g0 = COLUMN ? 0 : (log_n - LOG_S);
 
@@ -376,6 +358,67 @@ __global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
     }
 }
 
+// XCD-aware order: workgroup b runs on XCD b % 8, so when the grid is a
+// multiple of 8 each XCD gets a contiguous range of tiles (the column tiles
+// that share DRAM rows, and one modulus' twiddles, stay in one L2)
+__device__ __forceinline__ int xcd_block(int xcd) {
+    if (xcd && (gridDim.x & 7) == 0) return (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+    return (int)blockIdx.x;
+}
+
+// strided batch: limb l of polynomial y of batch entry bi at src + bi*sB + y*sY + l*sL
+template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE = false>
+__global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
+    k_ntt_pass(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+               const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
+               const kbfv_tw *__restrict__ w1_tab, const uint64_t *__restrict__ mods, int mod0, int limbs,
+               int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL, int xcd) {
+    using G = Geo<LOG_S, COLUMN, LOG_T>;
+    constexpr int groups = (1 << (LOG_N - LOG_S)) / G::C;  // workgroups per limb
+    __shared__ uint64_t tile[G::WORDS];
+    int bid = xcd_block(xcd);
+    const int grp0 = bid % groups;
+    bid /= groups;
+    const int y = bid % polys;
+    bid /= polys;
+    const int bi = bid % batch;
+    const int l = bid / batch;
+    pass_body<LOG_N, LOG_S, LOG_T, COLUMN, FWD, NOSCALE>(tile, src + bi * sB + y * sY + l * sL,
+                                                         dst + bi * dB + y * dY + l * dL, tw, ninv_tab, w1_tab,
+                                                         mods, mod0 + l, grp0);
+}
+
+// ragged batch: descriptor i = {src, dst, s_limb, d_limb, mod0, limbs} (int64 words),
+// limb_start[i] = the first global limb of descriptor i; the second pass (IN_DST)
+// reads and writes dst
+template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD>
+__global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
+    k_ntt_pass_desc(const int64_t *__restrict__ desc, const int32_t *__restrict__ limb_start, int count,
+                    int in_dst, const kbfv_tw *__restrict__ tw, const kbfv_tw *__restrict__ ninv_tab,
+                    const uint64_t *__restrict__ mods, int xcd) {
+    using G = Geo<LOG_S, COLUMN, LOG_T>;
+    constexpr int groups = (1 << (LOG_N - LOG_S)) / G::C;
+    __shared__ uint64_t tile[G::WORDS];
+    const int bid = xcd_block(xcd);
+    const int grp0 = bid % groups;
+    const int gl = bid / groups;
+    // the last descriptor whose first limb is at or before gl (skips empty ones)
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (limb_start[mid] <= gl)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int64_t *d = desc + 6 * (long)lo;
+    const long l = gl - limb_start[lo];
+    uint64_t *dst = (uint64_t *)d[1];
+    const uint64_t *in = in_dst ? dst + l * d[3] : (const uint64_t *)d[0] + l * d[2];
+    pass_body<LOG_N, LOG_S, LOG_T, COLUMN, FWD, false>(tile, in, dst + l * d[3], tw, ninv_tab, ninv_tab, mods,
+                                                       (int)d[4] + (int)l, grp0);
+}
+
 struct pass_args {
     const kbfv_context *c;
     const uint64_t *src;
@@ -396,9 +439,9 @@ int launch_pass(const pass_args &a) {
         const long groups = (1L << (LOG_N - LOG_S)) / G::C;
         const long blocks = groups * a.limbs * a.polys * a.batch;
         KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
-        hipLaunchKernelGGL((k_ntt_pass<LOG_N, LOG_S, LOG_T, COLUMN, FWD, NOSCALE>), dim3((unsigned)blocks), dim3(G::T),
-                           0, a.st, a.src, a.dst, a.c->t.tw, a.c->t.ninv, a.c->t.inv_w1, a.c->t.mod, a.mod0, a.limbs,
-                           a.polys, a.batch, a.sB, a.sY, a.sL, a.dB, a.dY, a.dL, a.xcd);
+        hipLaunchKernelGGL((k_ntt_pass<LOG_N, LOG_S, LOG_T, COLUMN, FWD, NOSCALE>), dim3((unsigned)blocks), dim3(G::T), 0, a.st, a.src, a.dst, a.c->t.tw, a.c->t.ninv,
+                           a.c->t.inv_w1, a.c->t.mod, a.mod0, a.limbs, a.polys, a.batch, a.sB, a.sY, a.sL, a.dB, a.dY,
+                           a.dL, a.xcd);
         KB_LAUNCH_CHECK();
         return KBFV_OK;
     }
@@ -636,4 +679,79 @@ extern "C" int kbfv_ntt_autotune(kbfv_context *c, int dir, int mod0, int limbs, 
     if (best) *best = best_plan;
     if (best_ms) *best_ms = best_total;
     return KBFV_OK;
+}
+
+// ---------------------------------------------------------------- ragged batches
+namespace {
+template <int LOG_N, bool COLUMN, bool FWD>
+int launch_desc(const kbfv_context *c, const int64_t *desc, const int32_t *start, int count, int total, int in_dst,
+                hipStream_t st) {
+    constexpr int LOG_S = COLUMN ? col_log_default(LOG_N) : LOG_N - col_log_default(LOG_N);
+    using G = Geo<LOG_S, COLUMN, 12>;
+    const long blocks = (long)total * ((1L << (LOG_N - LOG_S)) / G::C);
+    KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
+    hipLaunchKernelGGL((k_ntt_pass_desc<LOG_N, LOG_S, 12, COLUMN, FWD>), dim3((unsigned)blocks), dim3(G::T), 0, st,
+                       desc, start, count, in_dst, c->t.tw, c->t.ninv, c->t.mod, 1);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+template <bool COLUMN, bool FWD>
+int launch_desc_n(const kbfv_context *c, const int64_t *desc, const int32_t *start, int count, int total,
+                  int in_dst, hipStream_t st) {
+    switch (c->log_n) {
+        case 12: return launch_desc<12, COLUMN, FWD>(c, desc, start, count, total, in_dst, st);
+        case 13: return launch_desc<13, COLUMN, FWD>(c, desc, start, count, total, in_dst, st);
+        case 14: return launch_desc<14, COLUMN, FWD>(c, desc, start, count, total, in_dst, st);
+        case 15: return launch_desc<15, COLUMN, FWD>(c, desc, start, count, total, in_dst, st);
+        case 16: return launch_desc<16, COLUMN, FWD>(c, desc, start, count, total, in_dst, st);
+        default:
+            kb_set_error("unsupported ring degree 2^%d", c->log_n);
+            return KBFV_EINVAL;
+    }
+}
+}  // namespace
+
+extern "C" int kbfv_ntt_desc_upload(const kbfv_context *c, const int64_t *host_desc, int count, int64_t *dev_desc,
+                                    int32_t *dev_limb_start, int *total_limbs, void *stream) {
+    KB_REQUIRE(c && host_desc && dev_desc && dev_limb_start && total_limbs, "null argument");
+    KB_REQUIRE(count >= 1 && count <= (1 << 20), "descriptor count out of range");
+    std::vector<int32_t> start((size_t)count + 1);
+    long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const int64_t *d = host_desc + 6 * (size_t)i;
+        const int64_t mod0 = d[4], limbs = d[5];
+        KB_REQUIRE(limbs >= 0 && mod0 >= 0 && mod0 + limbs <= c->nmod, "descriptor %d: modulus range out of bounds", i);
+        KB_REQUIRE(limbs == 0 || (d[0] && d[1]), "descriptor %d: null polynomial", i);
+        KB_REQUIRE(limbs == 0 || mod0 + limbs - 1 != c->oPlain || c->batching,
+                   "descriptor %d: plain modulus has no 2N-th root", i);
+        KB_REQUIRE(limbs <= 1 || (d[2] >= c->n && d[3] >= c->n), "descriptor %d: limbs overlap", i);
+        start[i] = (int32_t)total;
+        total += limbs;
+        KB_REQUIRE(total * c->n < (1L << 40), "ragged batch too large");
+    }
+    KB_REQUIRE(total > 0, "empty NTT");
+    start[count] = (int32_t)total;
+    hipStream_t st = (hipStream_t)stream;
+    KB_HIP(hipMemcpyAsync(dev_desc, host_desc, (size_t)count * 6 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    KB_HIP(hipMemcpyAsync(dev_limb_start, start.data(), start.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    // the host vector goes out of scope: wait for the copies to leave it
+    KB_HIP(hipStreamSynchronize(st));
+    *total_limbs = (int)total;
+    return KBFV_OK;
+}
+
+extern "C" int kbfv_ntt_desc(const kbfv_context *c, int dir, const int64_t *dev_desc, const int32_t *dev_limb_start,
+                             int count, int total_limbs, void *stream) {
+    KB_REQUIRE(c && dev_desc && dev_limb_start, "null argument");
+    KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
+    KB_REQUIRE(count >= 1 && total_limbs >= 1, "empty NTT");
+    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
+    hipStream_t st = (hipStream_t)stream;
+    if (dir > 0) {
+        KB_TRY((launch_desc_n<true, true>(c, dev_desc, dev_limb_start, count, total_limbs, 0, st)));
+        return launch_desc_n<false, true>(c, dev_desc, dev_limb_start, count, total_limbs, 1, st);
+    }
+    KB_TRY((launch_desc_n<false, false>(c, dev_desc, dev_limb_start, count, total_limbs, 0, st)));
+    return launch_desc_n<true, false>(c, dev_desc, dev_limb_start, count, total_limbs, 1, st);
 }
