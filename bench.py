@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--serving-qps", type=float, default=3000.0)
     ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in NTT plans")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the HomMult + RGSW external-product mix at the same shape")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernel and a 4 GiB copy (for counter runs)")
     return ap.parse_args()
@@ -230,6 +232,8 @@ def main():
         "roofline": roof,
         "ntt_autotune": tuned,
     }
+    if not args.no_mixed:
+        result["mixed_ext_product"] = mixed_c4(ctx, sk, rlk, ct1, ct2, out, ws, m1, stream)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, ct1, ct2, tuple(rlk.shape))
     del ws, out, ct1, ct2, rlk
@@ -238,6 +242,58 @@ def main():
     if not args.no_serving:
         result["serving"] = serving_mix(args.serving_qps, args.serving_seconds)
     print(json.dumps(result), flush=True)
+
+
+def mixed_c4(ctx, sk, rlk, ct1, ct2, out, ws, m1, stream, bits=20):
+    """The batch's HomMult+relin mixed with as many RGSW x RLWE external
+    products (gadget base 2^bits over each Q limb, one RGSW of X^3 shared by
+    the batch, applied to the first input batch): each alone and both in one
+    timed region; the first product must decrypt to m1[0] * X^3."""
+    import numpy as np
+    import torch
+    import kunpbfv_amd as kb
+    B, N = ct1.shape[0], ctx.n
+    mg = np.zeros(N, dtype=np.uint64)
+    mg[3] = 1
+    rgsw = ctx.rgsw_encrypt(bytes([6]) * 32, sk, kb.to_device(mg), bits)
+    xout = ctx.empty(*ct1.shape)
+    xws = torch.empty(kb.lib.kbfv_external_product_workspace_bytes(ctx.h, B, bits), dtype=torch.uint8,
+                      device="cuda")
+
+    def ext():
+        kb.check(kb.lib.kbfv_external_product(ctx.h, kb.ptr(rgsw), kb.ptr(ct1), kb.ptr(xout), B, bits,
+                                              kb.ptr(xws), xws.numel(), kb.stream_ptr(stream)))
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    ext()
+    torch.cuda.synchronize()
+    dec = kb.to_host(ctx.decrypt(sk, xout[0:1].contiguous()))[0]
+    want = np.zeros(N, dtype=np.int64)
+    want[3:] = m1[0, :-3].astype(np.int64)
+    want[:3] = -m1[0, -3:].astype(np.int64)
+    ok = bool(np.array_equal(dec.astype(np.int64) % ctx.plain, want % ctx.plain))
+    t_ext = timed(ext)
+    t_mult = timed(lambda: ctx.mult_relin(rlk, ct1, ct2, out=out, ws=ws))
+
+    def both():
+        ctx.mult_relin(rlk, ct1, ct2, out=out, ws=ws)
+        ext()
+
+    t_mix = timed(both)
+    del rgsw, xws, xout
+    torch.cuda.empty_cache()
+    return {"workload": f"{B} HomMult+relin + {B} RGSW x RLWE external products (gadget base 2^{bits}, "
+                        f"{ctx.qn} x {60 // bits} digits per component), N=2^{ctx.log_n}",
+            "ext_product_ms": round(t_ext, 3), "ext_products_per_s": round(B / (t_ext / 1e3), 2),
+            "hommult_ms": round(t_mult, 3), "mixed_ms": round(t_mix, 3),
+            "mixed_ops_per_s": round(2 * B / (t_mix / 1e3), 2), "ext_product_decrypts_to_shift": ok}
 
 
 def graph_c3(stream, batch=64, reps=5):
