@@ -69,6 +69,10 @@ int kb_xsalsa20(const uint8_t key[32], const uint8_t nonce[24], uint64_t block0,
                 hipStream_t st);
 
 // extprod.hip
+// forward NTT of the gadget digit polynomials of rows r0..r0+rc-1 with the
+// digit split fused into the first pass (digits must stay below 8 q_min)
+int kb_digit_ntt(const kbfv_context *c, const uint64_t *coef, uint64_t *dig, int batch, int bits, int nd, int r0,
+                 int rc, hipStream_t st);
 int kb_digit_decompose(const kbfv_context *c, const uint64_t *coef, uint64_t *dig, int batch, int base_bits,
                        int nd, int r0, int rc, hipStream_t st);
 // out [batch][2][qn][N] += sum over rc rows of dig [batch][rc][qn][N] * rgsw_rows [rc][2][qn][N]

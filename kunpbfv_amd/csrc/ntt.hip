@@ -29,6 +29,7 @@
 // (direction, transform-count bucket); kbfv_ntt_autotune times the candidates
 // on the caller's stream and stores the fastest.
 #include <algorithm>
+#include <type_traits>
 
 #include "kbfv_internal.h"
 
@@ -157,13 +158,14 @@ __device__ __forceinline__ void round_twiddles(kbfv_tw *tws, const int lam0, con
 // 128-byte runs); it writes them back to the tile or, when GOUT, straight to
 // global memory with the final reduction of the last pass (FIN).
 template <int LR, int LOG_S, bool COLUMN, int LOG_T, bool FWD, bool NINV, bool GIN = false,
-          bool GOUT = false, bool FIN = false, bool PRE = false, bool W1 = false>
+          bool GOUT = false, bool FIN = false, bool PRE = false, bool W1 = false, bool DIG = false>
 __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const kbfv_tw *tab,
                                           const int log_n, const int g0, const int grp0,
                                           const int stride, const uint64_t q, const kbfv_tw ninv,
                                           const uint64_t *__restrict__ gin = nullptr,
                                           uint64_t *__restrict__ gout = nullptr,
-                                          const kbfv_tw (*pre)[(1 << LR) - 1] = nullptr) {
+                                          const kbfv_tw (*pre)[(1 << LR) - 1] = nullptr,
+                                          const int dshift = 0, const uint64_t dmask = ~0ULL) {
     using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, RR = 1 << LR;
     constexpr int NG = C * (S >> LR);
@@ -190,9 +192,11 @@ __device__ __forceinline__ void ntt_round(uint64_t *tile, const int lam0, const 
         uint64_t x[RR];
 #pragma unroll
         for (int i = 0; i < RR; ++i) {
-            if constexpr (GIN)
+            if constexpr (GIN) {
                 x[i] = gin[posc + (kbase + i * d) * pstep];
-            else
+                // DIG: the input is one base-2^w digit of the loaded word
+                if constexpr (DIG) x[i] = (x[i] >> dshift) & dmask;
+            } else
                 x[i] = tile[G::addr(c, kbase + i * d)];
         }
         if constexpr (FWD) {
@@ -278,11 +282,14 @@ __device__ __forceinline__ void stage_in(uint64_t *tile, const uint64_t *__restr
 }
 
 // one workgroup's tile of one limb: `in` / `out` address the limb, m is its modulus
-template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE>
+// DIG (forward column pass only): the input value is (word >> dshift) & dmask
+template <int LOG_N, int LOG_S, int LOG_T, bool COLUMN, bool FWD, bool NOSCALE, bool DIG = false>
 __device__ __forceinline__ void pass_body(uint64_t *tile, const uint64_t *__restrict__ in,
                                           uint64_t *__restrict__ out, const kbfv_tw *__restrict__ tw,
                                           const kbfv_tw *__restrict__ ninv_tab, const kbfv_tw *__restrict__ w1_tab,
-                                          const uint64_t *__restrict__ mods, const int m, const int grp0) {
+                                          const uint64_t *__restrict__ mods, const int m, const int grp0,
+                                          const int dshift = 0, const uint64_t dmask = ~0ULL) {
+    static_assert(!DIG || (COLUMN && FWD), "digit input only on the forward column pass");
     constexpr int log_n = LOG_N;
     using G = Geo<LOG_S, COLUMN, LOG_T>;
     constexpr int S = G::S, C = G::C, T = G::T;
@@ -305,8 +312,8 @@ __device__ __forceinline__ void pass_body(uint64_t *tile, const uint64_t *__rest
     // touch 128-byte runs in both layouts; the stride-1 rounds of the row pass
     // go through the LDS tile for both their global load and store.
     if constexpr (FWD) {
-        ntt_round<LR0, LOG_S, COLUMN, LOG_T, true, false, true>(tile, 0, tab, log_n, g0, grp0, stride, q, ninv,
-                                                                in);
+        ntt_round<LR0, LOG_S, COLUMN, LOG_T, true, false, true, false, false, false, false, DIG>(
+            tile, 0, tab, log_n, g0, grp0, stride, q, ninv, in, nullptr, nullptr, dshift, dmask);
         if constexpr (COLUMN) {
             ntt_round<LR1, LOG_S, COLUMN, LOG_T, true, false, false, true, LAST>(tile, LR0, tab, log_n, g0, grp0,
                                                                                  stride, q, ninv, nullptr, out);
@@ -475,6 +482,35 @@ int launch_pass_n(int log_s, int log_t, const pass_args &a) {
     }
 }
 
+// forward column pass of the external product's digit polynomials, the digit
+// split fused into the load: row r = (h, i, j) of the gadget takes digit j of
+// limb i of component h of the coefficient-form ciphertext, reduced to every
+// Q limb l.  coef [batch][2][qn][N] -> dig [batch][rc][qn][N] (rows r0..r0+rc-1);
+// the target limb varies fastest, so the qn workgroups that read one source
+// tile run back to back on one XCD and share it through L2
+template <int LOG_N, int LOG_S, int LOG_T>
+__global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
+    k_ntt_digit_col(const uint64_t *__restrict__ coef, uint64_t *__restrict__ dig, const kbfv_tw *__restrict__ tw,
+                    const uint64_t *__restrict__ mods, int oQ, int qn, int nd, int bits, int r0, int rc, int xcd) {
+    using G = Geo<LOG_S, true, LOG_T>;
+    constexpr int groups = (1 << (LOG_N - LOG_S)) / G::C;
+    constexpr long n = 1L << LOG_N;
+    __shared__ uint64_t tile[G::WORDS];
+    int bid = xcd_block(xcd);
+    const int l = bid % qn;
+    bid /= qn;
+    const int grp0 = bid % groups;
+    bid /= groups;
+    const int y = bid % rc;
+    const long bt = bid / rc;
+    const int row = r0 + y;
+    const long L = qn * n;
+    const uint64_t *in = coef + bt * 2 * L + (long)(row / nd) * n;  // (h*qn + i)*N = h*L + i*N
+    uint64_t *out = dig + (bt * rc + y) * L + (long)l * n;
+    pass_body<LOG_N, LOG_S, LOG_T, true, true, false, true>(tile, in, out, tw, nullptr, nullptr, mods, oQ + l, grp0,
+                                                            bits * (row % nd), (1ULL << bits) - 1);
+}
+
 // ---------------------------------------------------------------- plans
 struct plan_fields {
     int col_s, col_t, row_t, col_x, row_x;
@@ -559,6 +595,58 @@ int kb_intt_noscale(const kbfv_context *c, const uint64_t *src, uint64_t *dst, i
     KB_TRY(check_args(c, src, dst, mod0, limbs, polys, batch));
     const uint32_t plan = plan_for(c, -1, (long)limbs * polys * batch);
     return ntt_with_plan(c, -1, true, plan, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, st);
+}
+
+namespace {
+template <int LOG_N>
+int launch_digit_col(const kbfv_context *c, int log_t, const uint64_t *coef, uint64_t *dig, int batch, int bits, int nd,
+                     int r0, int rc, hipStream_t st) {
+    constexpr int LOG_S = col_log_default(LOG_N);
+    auto go = [&](auto tag) -> int {
+        constexpr int LT = decltype(tag)::value;
+        if constexpr (!pass_ok(LOG_N, LOG_S, LT, true)) {
+            kb_set_error("digit NTT: tile 2^%d does not fit N=2^%d", LT, LOG_N);
+            return KBFV_EINVAL;
+        } else {
+            using G = Geo<LOG_S, true, LT>;
+            const long blocks = (long)c->qn * ((1L << (LOG_N - LOG_S)) / G::C) * rc * batch;
+            KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
+            hipLaunchKernelGGL((k_ntt_digit_col<LOG_N, LOG_S, LT>), dim3((unsigned)blocks), dim3(G::T), 0, st, coef,
+                               dig, c->t.tw, c->t.mod, c->oQ, c->qn, nd, bits, r0, rc, 1);
+            KB_LAUNCH_CHECK();
+            return KBFV_OK;
+        }
+    };
+    switch (log_t) {
+        case 11: return go(std::integral_constant<int, 11>{});
+        case 12: return go(std::integral_constant<int, 12>{});
+        default: return go(std::integral_constant<int, 13>{});
+    }
+}
+}  // namespace
+
+int kb_digit_ntt(const kbfv_context *c, const uint64_t *coef, uint64_t *dig, int batch, int bits, int nd, int r0,
+                 int rc, hipStream_t st) {
+    KB_REQUIRE(c && coef && dig && batch > 0 && rc > 0 && nd > 0 && bits >= 1 && bits <= 62, "bad digit NTT arguments");
+    KB_REQUIRE(c->log_n >= 12 && c->log_n <= 16, "NTT supports 2^12..2^16");
+    const int qn = c->qn, log_n = c->log_n;
+    const long N = c->n, L = (long)qn * N;
+    // the fused split runs the built-in split; it keeps the plan's tiles and
+    // row order when the plan uses that split too
+    const plan_fields f = decode_plan(log_n, plan_for(c, +1, (long)qn * rc * batch));
+    const bool same = f.col_s == col_log_default(log_n);
+    const int col_t = same ? f.col_t : 12, row_t = same ? f.row_t : 12, row_x = same ? f.row_x : 1;
+    int rc2;
+    switch (log_n) {
+        case 12: rc2 = launch_digit_col<12>(c, col_t, coef, dig, batch, bits, nd, r0, rc, st); break;
+        case 13: rc2 = launch_digit_col<13>(c, col_t, coef, dig, batch, bits, nd, r0, rc, st); break;
+        case 14: rc2 = launch_digit_col<14>(c, col_t, coef, dig, batch, bits, nd, r0, rc, st); break;
+        case 15: rc2 = launch_digit_col<15>(c, col_t, coef, dig, batch, bits, nd, r0, rc, st); break;
+        default: rc2 = launch_digit_col<16>(c, col_t, coef, dig, batch, bits, nd, r0, rc, st); break;
+    }
+    if (rc2) return rc2;
+    const pass_args second{c, dig, dig, c->oQ, qn, rc, batch, rc * L, L, N, rc * L, L, N, row_x, st};
+    return launch_pass_n<false, true>(log_n - col_log_default(log_n), row_t, second);
 }
 
 // ---------------------------------------------------------------- plan ABI and autotuner

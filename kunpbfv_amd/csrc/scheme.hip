@@ -856,10 +856,19 @@ extern "C" int kbfv_external_product(const kbfv_context *c, const uint64_t *rgsw
     hipStream_t s = S(st);
     KB_TRY(kb_ntt(c, -1, ct, coef, c->oQ, qn, 2, batch, 2 * L, L, N, 2 * L, L, N, s));
     KB_TRY(fill_zero(out, 2 * L * batch, s));
+    // digits below 8 q_min are valid lazy NTT inputs: split them inside the
+    // NTT's first pass instead of writing them to every limb first
+    uint64_t qmin = ~0ULL;
+    for (int i = 0; i < qn; ++i) qmin = std::min(qmin, c->mod[c->oQ + i]);
+    const bool fused = base_bits <= 62 && ((1ULL << base_bits) >> 3) <= qmin;
     for (int r0 = 0; r0 < rows; r0 += 8) {
         const int rc = std::min(8, rows - r0);
-        KB_TRY(kb_digit_decompose(c, coef, dig, batch, base_bits, nd, r0, rc, s));
-        KB_TRY(kb_ntt(c, +1, dig, dig, c->oQ, qn, rc, batch, (long)rc * L, L, N, (long)rc * L, L, N, s));
+        if (fused) {
+            KB_TRY(kb_digit_ntt(c, coef, dig, batch, base_bits, nd, r0, rc, s));
+        } else {
+            KB_TRY(kb_digit_decompose(c, coef, dig, batch, base_bits, nd, r0, rc, s));
+            KB_TRY(kb_ntt(c, +1, dig, dig, c->oQ, qn, rc, batch, (long)rc * L, L, N, (long)rc * L, L, N, s));
+        }
         KB_TRY(kb_gadget_mac(c, dig, rgsw + (long)r0 * 2 * L, out, batch, rc, s));
     }
     return KBFV_OK;

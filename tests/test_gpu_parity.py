@@ -218,11 +218,14 @@ def test_scheme_bit_exact_with_oracle(pair):
     assert np.array_equal(kb.to_host(one)[0], ph[1])
 
 
-def test_external_product_matches_oracle(pair):
+@pytest.mark.parametrize("base", [20, 40])
+def test_external_product_matches_oracle(pair, base):
+    """base 20 splits the digits inside the NTT's first pass on every set;
+    base 40 takes the separate digit kernel where a digit can reach 8 q (the
+    36-bit chain of N=4096) and the fused pass elsewhere."""
     import kunpbfv_amd as kb
     ctx, oc = pair
     sk, pk, rlk, o_sk, o_pk, o_rlk = _keys(ctx, oc)
-    base = 20
     rng = np.random.default_rng(23)
     mg = np.zeros(ctx.n, dtype=np.uint64)
     mg[3] = 1  # X^3
@@ -236,6 +239,8 @@ def test_external_product_matches_oracle(pair):
     cth = kb.to_host(ct)
     for b in range(2):
         assert np.array_equal(oh[b], oc.external_product(o_rg, cth[b], base)), b
+    if base > 20:
+        return  # larger digits: the noise bound is not the point here
     dec = kb.to_host(ctx.decrypt(sk, out))
     for b in range(2):
         want = np.zeros(ctx.n, dtype=object)
