@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--serving-qps", type=float, default=3000.0)
     ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in NTT plans")
+    ap.add_argument("--no-ntt-alone", action="store_true",
+                    help="skip the NTT-alone measurement at N=2^14, 8 limbs, 256 polynomials")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the HomMult + RGSW external-product mix at the same shape")
     ap.add_argument("--roofline-only", action="store_true",
@@ -237,6 +239,8 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, ct1, ct2, tuple(rlk.shape))
     del ws, out, ct1, ct2, rlk
+    if not args.no_ntt_alone:
+        result["ntt_n16384_l8_b256"] = ntt_alone(stream)
     if not args.no_graph:
         result["graph_n32768_b64"] = graph_c3(stream)
     if not args.no_serving:
@@ -294,6 +298,72 @@ def mixed_c4(ctx, sk, rlk, ct1, ct2, out, ws, m1, stream, bits=20):
             "ext_product_ms": round(t_ext, 3), "ext_products_per_s": round(B / (t_ext / 1e3), 2),
             "hommult_ms": round(t_mult, 3), "mixed_ms": round(t_mix, 3),
             "mixed_ops_per_s": round(2 * B / (t_mix / 1e3), 2), "ext_product_decrypts_to_shift": ok}
+
+
+def ntt_alone(stream, polys=256, reps=10):
+    """The batched NTT kernel alone: N=2^14, 8 x 60-bit primes (1 mod 2^17),
+    256 polynomials of seeded uniform residues, forward then inverse with the
+    autotuned plans; algorithmic bytes = every limb read and written once per
+    transform.  The inverse must return the input bit for bit."""
+    import numpy as np
+    import torch
+    import kunpbfv_amd as kb
+    ctx = kb.Context.chain60(14, 8, 1, 4, 2, 65537)
+    mod0, limbs = ctx.oQ, ctx.qn
+    rng = np.random.default_rng(2)
+    x = np.empty((polys, limbs, ctx.n), dtype=np.uint64)
+    for i in range(limbs):
+        x[:, i, :] = rng.integers(0, int(ctx.mod[mod0 + i]), size=(polys, ctx.n), dtype=np.uint64)
+    xd = kb.to_device(x)
+    plans = {d: ctx.autotune_ntt(d, mod0, limbs, polys, reps=3)[0] for d in (+1, -1)}
+    yd = xd.clone()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ctx.ntt(yd, yd, +1, mod0, limbs, polys=polys)
+    ctx.ntt(yd, yd, -1, mod0, limbs, polys=polys)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(reps):
+        ctx.ntt(yd, yd, +1, mod0, limbs, polys=polys)
+    ev[1].record(stream)
+    for _ in range(reps):
+        ctx.ntt(yd, yd, -1, mod0, limbs, polys=polys)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    t_f = ev[0].elapsed_time(ev[1]) / reps
+    t_i = ev[1].elapsed_time(ev[2]) / reps
+    exact = bool(torch.equal(yd, xd))
+    nbytes = 2 * polys * limbs * ctx.n * 8  # read + write of every limb, per transform
+    # the single-pass plan (one workgroup per limb) for comparison
+    bucket = kb.lib.kbfv_ntt_plan_bucket(limbs * polys)
+    ctx.set_ntt_plan(+1, bucket, ctx.log_n | 1 << 12)
+    ev[0].record(stream)
+    for _ in range(reps):
+        ctx.ntt(yd, yd, +1, mod0, limbs, polys=polys)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    t_single = ev[0].elapsed_time(ev[1]) / reps
+    ctx.set_ntt_plan(+1, bucket, plans[+1])
+
+    def fields(p):
+        cs, ct, rt, cx, rx = kb.ntt_plan_fields(p, ctx.log_n)
+        return {"plan": hex(p), "passes": 1 if cs == ctx.log_n else 2, "split_log": [cs, ctx.log_n - cs],
+                "tile_log": [ct, rt]}
+
+    res = {"workload": f"negacyclic NTT, N=2^14, {limbs} x 60-bit primes, {polys} polynomials "
+                       f"({polys * limbs} limbs), forward then inverse, {reps} of each",
+           "forward_ms": round(t_f, 4), "inverse_ms": round(t_i, 4),
+           "forward_gbs": round(nbytes / t_f / 1e6, 1), "inverse_gbs": round(nbytes / t_i / 1e6, 1),
+           "forward_frac": round(nbytes / t_f / 1e6 / HBM_PEAK_GBS, 4),
+           "inverse_frac": round(nbytes / t_i / 1e6 / HBM_PEAK_GBS, 4),
+           "per_launch_gbs": {"forward": round(nbytes * fields(plans[+1])["passes"] / t_f / 1e6, 1),
+                              "inverse": round(nbytes * fields(plans[-1])["passes"] / t_i / 1e6, 1)},
+           "single_pass_forward_ms": round(t_single, 4),
+           "bytes_per_transform": nbytes, "forward_plan": fields(plans[+1]), "inverse_plan": fields(plans[-1]),
+           "working_set_bytes": polys * limbs * ctx.n * 8,
+           "inverse_of_forward_is_identity": exact}
+    del xd, yd
+    torch.cuda.empty_cache()
+    return res
 
 
 def graph_c3(stream, batch=64, reps=5):

@@ -482,6 +482,76 @@ int launch_pass_n(int log_s, int log_t, const pass_args &a) {
     }
 }
 
+// single-pass transform of a whole limb (N <= 2^14: the limb and its row
+// padding fit in the 160 KB LDS): one HBM read and one write per limb instead
+// of two of each.  Register rounds: radix 16 first, the remaining log N - 4
+// levels in rounds of at most 4 (12: 4,4,4; 13: 4,3,3,3; 14: 4,4,3,3).
+__host__ __device__ constexpr int limb_rounds(int log_n) { return 1 + (log_n - 4 + 3) / 4; }
+__host__ __device__ constexpr int limb_round_log(int log_n, int k) {
+    if (k == 0) return 4;
+    const int r = log_n - 4, nr = (r + 3) / 4;
+    return r / nr + (k - 1 < r % nr ? 1 : 0);
+}
+__host__ __device__ constexpr int limb_round_lam(int log_n, int k) {
+    return k == 0 ? 0 : limb_round_lam(log_n, k - 1) + limb_round_log(log_n, k - 1);
+}
+
+template <int LOG_N, int K, bool FWD, bool NI>
+__device__ __forceinline__ void limb_rounds_mid(uint64_t *tile, const kbfv_tw *tab, uint64_t q, kbfv_tw ninv) {
+    // rounds 1 .. limb_rounds - 1, LDS to LDS: ascending (forward) / descending (inverse)
+    if constexpr (K >= 1 && K < limb_rounds(LOG_N)) {
+        constexpr int KK = FWD ? K : limb_rounds(LOG_N) - K;
+        ntt_round<limb_round_log(LOG_N, KK), LOG_N, false, LOG_N, FWD, NI>(tile, limb_round_lam(LOG_N, KK), tab,
+                                                                         LOG_N, 0, 0, 1, q, ninv);
+        limb_rounds_mid<LOG_N, K + 1, FWD, NI>(tile, tab, q, ninv);
+    }
+}
+
+template <int LOG_N, bool FWD, bool NOSCALE = false>
+__global__ void __launch_bounds__((1 << LOG_N) >> LOG_R)
+    k_ntt_limb(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst, const kbfv_tw *__restrict__ tw,
+               const kbfv_tw *__restrict__ ninv_tab, const kbfv_tw *__restrict__ w1_tab,
+               const uint64_t *__restrict__ mods, int mod0, int limbs, int polys, int batch, long sB, long sY,
+               long sL, long dB, long dY, long dL, int xcd) {
+    static_assert(LOG_N >= 12 && LOG_N <= 14, "one limb per workgroup up to 2^14");
+    using G = Geo<LOG_N, false, LOG_N>;
+    constexpr int T = G::T;
+    __shared__ uint64_t tile[G::WORDS];
+    int bid = xcd_block(xcd);
+    const int y = bid % polys;
+    bid /= polys;
+    const int bi = bid % batch;
+    const int l = bid / batch;
+    constexpr int n = 1 << LOG_N;
+    const int m = mod0 + l;
+    const uint64_t q = mods[m];
+    const kbfv_tw *tab = tw + ((size_t)m * 2 + (FWD ? 0 : 1)) * n;
+    const uint64_t *in = src + bi * sB + y * sY + l * sL;
+    uint64_t *out = dst + bi * dB + y * dY + l * dL;
+    constexpr bool NI = !FWD && !NOSCALE;
+    const kbfv_tw ninv = NI ? ninv_tab[m] : (NOSCALE ? w1_tab[m] : kbfv_tw{0, 0});
+    if constexpr (FWD) {
+        // the radix-16 round of stride N/16 reads global memory in 128-byte runs
+        ntt_round<4, LOG_N, false, LOG_N, true, false, true>(tile, 0, tab, LOG_N, 0, 0, 1, q, ninv, in);
+        limb_rounds_mid<LOG_N, 1, true, false>(tile, tab, q, ninv);
+#pragma unroll
+        for (int j = 0; j < n / T; ++j) {
+            const int k = threadIdx.x + j * T;
+            uint64_t v = tile[G::addr(0, k)];
+            v = csub_sel(v, 0 - (q << 2));
+            v = csub_sel(v, 0 - (q << 1));
+            v = csub_sel(v, 0 - q);
+            out[k] = v;
+        }
+    } else {
+        stage_in<LOG_N, false, LOG_N>(tile, in, 0, 1);
+        __syncthreads();
+        limb_rounds_mid<LOG_N, 1, false, NI>(tile, tab, q, ninv);
+        ntt_round<4, LOG_N, false, LOG_N, false, NI, false, true, !NOSCALE, false, NOSCALE>(
+            tile, 0, tab, LOG_N, 0, 0, 1, q, ninv, nullptr, out);
+    }
+}
+
 // forward column pass of the external product's digit polynomials, the digit
 // split fused into the load: row r = (h, i, j) of the gadget takes digit j of
 // limb i of component h of the coefficient-form ciphertext, reduced to every
@@ -521,10 +591,14 @@ plan_fields decode_plan(int log_n, uint32_t p) {
     return {(int)(p & 15), (int)((p >> 4) & 15), (int)((p >> 8) & 15), (int)((p >> 12) & 1), (int)((p >> 13) & 1)};
 }
 
+// single-pass plan: column length = log N, no tiles, row order bit clear
+bool single_pass(int log_n, const plan_fields &f) { return f.col_s == log_n; }
+
 bool plan_ok(int log_n, uint32_t p) {
     if (p == 0) return true;
     if (p >> 14) return false;
     const plan_fields f = decode_plan(log_n, p);
+    if (single_pass(log_n, f)) return log_n <= 14 && f.col_t == 0 && f.row_t == 0 && f.row_x == 0;
     const int d = f.col_s - col_log_default(log_n);
     return d >= -1 && d <= 1 && pass_ok(log_n, f.col_s, f.col_t, true) &&
            pass_ok(log_n, log_n - f.col_s, f.row_t, false);
@@ -553,10 +627,38 @@ int check_args(const kbfv_context *c, const uint64_t *src, const uint64_t *dst, 
 }
 
 // one transform with an explicit plan; NOSCALE: inverse without n^-1 and the final reduction
+template <int LOG_N, bool FWD, bool NOSCALE>
+int launch_limb(const pass_args &a) {
+    const long blocks = (long)a.limbs * a.polys * a.batch;
+    KB_REQUIRE(blocks < (1L << 31), "NTT grid too large");
+    hipLaunchKernelGGL((k_ntt_limb<LOG_N, FWD, NOSCALE>), dim3((unsigned)blocks), dim3((1 << LOG_N) >> LOG_R), 0, a.st,
+                       a.src, a.dst, a.c->t.tw, a.c->t.ninv, a.c->t.inv_w1, a.c->t.mod, a.mod0, a.limbs, a.polys,
+                       a.batch, a.sB, a.sY, a.sL, a.dB, a.dY, a.dL, a.xcd);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
+
+template <bool FWD, bool NOSCALE>
+int launch_limb_n(const pass_args &a) {
+    switch (a.c->log_n) {
+        case 12: return launch_limb<12, FWD, NOSCALE>(a);
+        case 13: return launch_limb<13, FWD, NOSCALE>(a);
+        case 14: return launch_limb<14, FWD, NOSCALE>(a);
+        default:
+            kb_set_error("single-pass NTT needs N <= 2^14");
+            return KBFV_EINVAL;
+    }
+}
+
 int ntt_with_plan(const kbfv_context *c, int dir, bool noscale, uint32_t plan, const uint64_t *src, uint64_t *dst,
                   int mod0, int limbs, int polys, int batch, long sB, long sY, long sL, long dB, long dY, long dL,
                   hipStream_t st) {
     const plan_fields f = decode_plan(c->log_n, plan);
+    if (single_pass(c->log_n, f)) {
+        const pass_args a{c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, f.col_x, st};
+        if (dir > 0) return launch_limb_n<true, false>(a);
+        return noscale ? launch_limb_n<false, true>(a) : launch_limb_n<false, false>(a);
+    }
     const int row_s = c->log_n - f.col_s;
     pass_args first{c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, dir > 0 ? f.col_x : f.row_x, st};
     pass_args second{c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, dir > 0 ? f.row_x : f.col_x, st};
@@ -659,6 +761,11 @@ extern "C" int kbfv_ntt_plan_candidates(int log_n, uint32_t *out, int cap) {
     KB_REQUIRE(log_n >= 12 && log_n <= 16, "NTT supports 2^12..2^16");
     KB_REQUIRE(cap >= 0 && (out || cap == 0), "null output");
     int k = 0;
+    for (int cx = 0; cx <= 1 && log_n <= 14; ++cx) {
+        const uint32_t p = encode_plan(log_n, 0, 0, cx, 0);
+        if (k < cap) out[k] = p;
+        ++k;
+    }
     for (int d = -1; d <= 1; ++d)
         for (int ct = 11; ct <= 13; ++ct)
             for (int rt = 11; rt <= 13; ++rt)
@@ -757,6 +864,25 @@ extern "C" int kbfv_ntt_autotune(kbfv_context *c, int dir, int mod0, int limbs, 
         if (rc == KBFV_OK && bct && brt && bc + br < best_total) {
             best_total = bc + br;
             best_plan = encode_plan(cs, bct, brt, bcx, brx);
+        }
+    }
+    // one launch per transform where the limb fits a workgroup's LDS
+    for (int x = 0; x <= 1 && rc == KBFV_OK && log_n <= 14; ++x) {
+        const long N = c->n;
+        const pass_args a{c, scratch, scratch, mod0, limbs, polys, 1, 0, (long)limbs * N, N, 0, (long)limbs * N, N, x,
+                          st};
+        auto run = [&]() -> int { return dir > 0 ? launch_limb_n<true, false>(a) : launch_limb_n<false, false>(a); };
+        rc = run();
+        if (rc == KBFV_OK) rc = hipEventRecord(e0, st) == hipSuccess ? KBFV_OK : KBFV_EHIP;
+        for (int r = 0; r < reps && rc == KBFV_OK; ++r) rc = run();
+        if (rc == KBFV_OK) rc = hipEventRecord(e1, st) == hipSuccess ? KBFV_OK : KBFV_EHIP;
+        if (rc == KBFV_OK) rc = hipEventSynchronize(e1) == hipSuccess ? KBFV_OK : KBFV_EHIP;
+        float t = 0;
+        if (rc == KBFV_OK) rc = hipEventElapsedTime(&t, e0, e1) == hipSuccess ? KBFV_OK : KBFV_EHIP;
+        if (rc == KBFV_EHIP) kb_set_error("NTT autotune: HIP event call failed");
+        if (rc == KBFV_OK && t / reps < best_total) {
+            best_total = t / reps;
+            best_plan = encode_plan(log_n, 0, 0, x, 0);
         }
     }
     (void)hipEventDestroy(e0);

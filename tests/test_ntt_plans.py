@@ -20,6 +20,8 @@ def test_candidates_are_exactly_the_valid_shapes(log_n):
     import kunpbfv_amd as kb
     got = kb.ntt_plan_candidates(log_n)
     want = set()
+    if log_n <= 14:  # single pass: the whole limb in one workgroup's LDS
+        want |= {log_n | cx << 12 for cx in (0, 1)}
     c0 = (log_n + 1) // 2
     for cs in (c0 - 1, c0, c0 + 1):
         for ct in (11, 12, 13):
@@ -33,6 +35,8 @@ def test_candidates_are_exactly_the_valid_shapes(log_n):
         assert kb.lib.kbfv_ntt_plan_valid(log_n, p) == 0
         f = kb.ntt_plan_fields(p, log_n)
         assert f[0] + (log_n - f[0]) == log_n
+        if f[0] == log_n:
+            assert f[1:3] == (0, 0) and f[4] == 0
     # the built-in plan (0) is the balanced split, 4096-element tiles, XCD order
     assert kb.lib.kbfv_ntt_plan_valid(log_n, 0) == 0
     builtin = c0 | 12 << 4 | 12 << 8 | 1 << 12 | 1 << 13
@@ -43,6 +47,11 @@ def test_large_rings_offer_radix32_splits():
     import kunpbfv_amd as kb
     splits = {p & 15 for p in kb.ntt_plan_candidates(16)}
     assert splits == {7, 8, 9}
+    # a limb of 2^15 (256 KiB) does not fit one workgroup's LDS: no single pass
+    assert 15 not in {p & 15 for p in kb.ntt_plan_candidates(15)}
+    assert 14 in {p & 15 for p in kb.ntt_plan_candidates(14)}
+    assert kb.lib.kbfv_ntt_plan_valid(15, 15) != 0
+    assert kb.lib.kbfv_ntt_plan_valid(14, 14 | 12 << 4) != 0   # a single pass has no tiles
     # a 2^9-point column pass needs 8 columns of 512: tiles of 2^12 or more
     assert not any((p & 15) == 9 and ((p >> 4) & 15) == 11 for p in kb.ntt_plan_candidates(16))
 
