@@ -126,6 +126,17 @@ int kbfv_ntt_desc_upload(const kbfv_context *ctx, const int64_t *host_desc, int 
                          int32_t *dev_limb_start, int *total_limbs, void *stream);
 int kbfv_ntt_desc(const kbfv_context *ctx, int dir, const int64_t *dev_desc, const int32_t *dev_limb_start,
                   int count, int total_limbs, void *stream);
+/* Element-wise add (op 0), sub (1) or mul (2) over a ragged batch, one launch
+ * (replaces per-level loops of poly_add/sub/mul batch kernels,
+ * include/poly_arithmetic.cuh:26-360).  Descriptor i is eight int64 words
+ * {a, b, out, s_a, s_b, s_out, mod0, limbs}: limb l of each operand at
+ * base + l*stride, modulus mod0 + l, canonical residues (out may alias a or
+ * b).  Upload validates and copies as kbfv_ntt_desc_upload does; the launch
+ * reads only device memory (graph-capturable). */
+int kbfv_poly_binop_desc_upload(const kbfv_context *ctx, const int64_t *host_desc, int count, int64_t *dev_desc,
+                                int32_t *dev_limb_start, int *total_limbs, void *stream);
+int kbfv_poly_binop_desc(const kbfv_context *ctx, int op, const int64_t *dev_desc, const int32_t *dev_limb_start,
+                         int count, int total_limbs, void *stream);
 
 /* ---------------------------------------------------------------- RNS kernels */
 

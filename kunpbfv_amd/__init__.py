@@ -44,6 +44,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "kbfv_ntt_desc_upload": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _ip, _vp]),
     "kbfv_ntt_desc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "kbfv_poly_binop_desc_upload": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _ip, _vp]),
+    "kbfv_poly_binop_desc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp]),
     "kbfv_poly_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_poly_sub": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "kbfv_poly_mul": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
@@ -376,6 +378,16 @@ class Context:
         check(lib.kbfv_ntt_desc(self.h, direction, ptr(batch.desc), ptr(batch.start), batch.count, batch.total,
                                 stream_ptr(stream)))
 
+    def ragged_binop(self, items, stream=None):
+        """Upload descriptors of a ragged element-wise batch: items are
+        (a, b, out, mod0, limbs) with [limbs][N] device tensors."""
+        return RaggedBinop(self, items, stream)
+
+    def binop_ragged(self, op, batch, stream=None):
+        """op 0 add, 1 sub, 2 mul over every descriptor of a RaggedBinop."""
+        check(lib.kbfv_poly_binop_desc(self.h, op, ptr(batch.desc), ptr(batch.start), batch.count, batch.total,
+                                       stream_ptr(stream)))
+
     def poly_op(self, op, a, b, out, mod0, limbs, polys=1, stream=None):
         fn = {"add": lib.kbfv_poly_add, "sub": lib.kbfv_poly_sub, "mul": lib.kbfv_poly_mul}[op]
         check(fn(self.h, ptr(a), ptr(b), ptr(out), mod0, limbs, polys, stream_ptr(stream)))
@@ -626,6 +638,27 @@ class RaggedBatch:
                                        C.byref(total), stream_ptr(stream)))
         self.count, self.total = len(items), total.value
         self._keep = [it[:2] for it in items]
+
+
+class RaggedBinop:
+    """Device descriptors of an element-wise batch over polynomials of
+    different RNS lengths (see kbfv_poly_binop_desc_upload)."""
+
+    def __init__(self, ctx, items, stream=None):
+        torch = _torch()
+        desc = np.zeros((len(items), 8), dtype=np.int64)
+        for i, (a, b, out, mod0, limbs) in enumerate(items):
+            for t in (a, b, out):
+                if limbs and t.numel() < limbs * ctx.n:
+                    raise KbfvError(f"descriptor {i}: tensor holds fewer than {limbs} limbs")
+            desc[i] = (a.data_ptr(), b.data_ptr(), out.data_ptr(), ctx.n, ctx.n, ctx.n, mod0, limbs)
+        self.desc = torch.empty(desc.size, dtype=torch.int64, device="cuda")
+        self.start = torch.empty(len(items) + 1, dtype=torch.int32, device="cuda")
+        total = C.c_int()
+        check(lib.kbfv_poly_binop_desc_upload(ctx.h, desc.ctypes.data, len(items), ptr(self.desc), ptr(self.start),
+                                              C.byref(total), stream_ptr(stream)))
+        self.count, self.total = len(items), total.value
+        self._keep = [it[:3] for it in items]
 
 
 class MultGraph:

@@ -969,3 +969,80 @@ extern "C" int kbfv_ntt_desc(const kbfv_context *c, int dir, const int64_t *dev_
     KB_TRY((launch_desc_n<false, false>(c, dev_desc, dev_limb_start, count, total_limbs, 0, st)));
     return launch_desc_n<true, false>(c, dev_desc, dev_limb_start, count, total_limbs, 1, st);
 }
+
+// ---------------------------------------------------------------- ragged element-wise
+namespace {
+constexpr int RB_BS = 256;
+
+// descriptor i = {a, b, out, s_a, s_b, s_out, mod0, limbs} (int64 words); op 0
+// add, 1 sub, 2 mul, on canonical residues; workgroup = 256 coefficients of one limb
+__global__ void __launch_bounds__(RB_BS)
+    k_binop_desc(const int64_t *__restrict__ desc, const int32_t *__restrict__ limb_start, int count, int op,
+                 const uint64_t *__restrict__ mods, const uint64_t *__restrict__ mu, int chunks) {
+    const int gl = (int)(blockIdx.x / chunks);
+    const int k = (int)(blockIdx.x % chunks) * RB_BS + threadIdx.x;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (limb_start[mid] <= gl)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int64_t *d = desc + 8 * (long)lo;
+    const long l = gl - limb_start[lo];
+    const int m = (int)d[6] + (int)l;
+    const uint64_t q = mods[m];
+    const uint64_t a = ((const uint64_t *)d[0])[l * d[3] + k];
+    const uint64_t b = ((const uint64_t *)d[1])[l * d[4] + k];
+    uint64_t r;
+    if (op == 0)
+        r = kb_addmod(a, b, q);
+    else if (op == 1)
+        r = kb_submod(a, b, q);
+    else
+        r = kb_mulmod(a, b, q, mu[2 * m], mu[2 * m + 1]);
+    ((uint64_t *)d[2])[l * d[5] + k] = r;
+}
+}  // namespace
+
+extern "C" int kbfv_poly_binop_desc_upload(const kbfv_context *c, const int64_t *host_desc, int count,
+                                           int64_t *dev_desc, int32_t *dev_limb_start, int *total_limbs,
+                                           void *stream) {
+    KB_REQUIRE(c && host_desc && dev_desc && dev_limb_start && total_limbs, "null argument");
+    KB_REQUIRE(count >= 1 && count <= (1 << 20), "descriptor count out of range");
+    std::vector<int32_t> start((size_t)count + 1);
+    long total = 0;
+    for (int i = 0; i < count; ++i) {
+        const int64_t *d = host_desc + 8 * (size_t)i;
+        const int64_t mod0 = d[6], limbs = d[7];
+        KB_REQUIRE(limbs >= 0 && mod0 >= 0 && mod0 + limbs <= c->nmod, "descriptor %d: modulus range out of bounds", i);
+        KB_REQUIRE(limbs == 0 || (d[0] && d[1] && d[2]), "descriptor %d: null polynomial", i);
+        KB_REQUIRE(limbs <= 1 || (d[3] >= c->n && d[4] >= c->n && d[5] >= c->n), "descriptor %d: limbs overlap", i);
+        start[i] = (int32_t)total;
+        total += limbs;
+        KB_REQUIRE(total * c->n < (1L << 40), "ragged batch too large");
+    }
+    KB_REQUIRE(total > 0, "empty batch");
+    start[count] = (int32_t)total;
+    hipStream_t st = (hipStream_t)stream;
+    KB_HIP(hipMemcpyAsync(dev_desc, host_desc, (size_t)count * 8 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    KB_HIP(hipMemcpyAsync(dev_limb_start, start.data(), start.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    KB_HIP(hipStreamSynchronize(st));  // the host vector goes out of scope
+    *total_limbs = (int)total;
+    return KBFV_OK;
+}
+
+extern "C" int kbfv_poly_binop_desc(const kbfv_context *c, int op, const int64_t *dev_desc,
+                                    const int32_t *dev_limb_start, int count, int total_limbs, void *stream) {
+    KB_REQUIRE(c && dev_desc && dev_limb_start, "null argument");
+    KB_REQUIRE(op >= 0 && op <= 2, "op must be 0 (add), 1 (sub) or 2 (mul)");
+    KB_REQUIRE(count >= 1 && total_limbs >= 1, "empty batch");
+    const int chunks = c->n / RB_BS;
+    const long blocks = (long)total_limbs * chunks;
+    KB_REQUIRE(blocks < (1L << 31), "grid too large");
+    hipLaunchKernelGGL(k_binop_desc, dim3((unsigned)blocks), dim3(RB_BS), 0, (hipStream_t)stream, dev_desc,
+                       dev_limb_start, count, op, c->t.mod, c->t.mu, chunks);
+    KB_LAUNCH_CHECK();
+    return KBFV_OK;
+}
