@@ -121,7 +121,9 @@ int kbfv_ntt_autotune(kbfv_context *ctx, int dir, int mod0, int limbs, int polys
  * against the context and copies them with their limb offsets (count + 1
  * int32) to caller device buffers, returning the total limb count;
  * kbfv_ntt_desc then reads only device memory, so it can be captured in a HIP
- * graph and replayed while the descriptors stay valid.  Built-in plan. */
+ * graph and replayed while the descriptors stay valid; pass it the count and
+ * the total the upload returned (limbs past the last descriptor are skipped).
+ * Built-in plan. */
 int kbfv_ntt_desc_upload(const kbfv_context *ctx, const int64_t *host_desc, int count, int64_t *dev_desc,
                          int32_t *dev_limb_start, int *total_limbs, void *stream);
 int kbfv_ntt_desc(const kbfv_context *ctx, int dir, const int64_t *dev_desc, const int32_t *dev_limb_start,

@@ -420,6 +420,7 @@ __global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
     }
     const int64_t *d = desc + 6 * (long)lo;
     const long l = gl - limb_start[lo];
+    if (l >= d[5]) return;  // a total past the uploaded one: uniform per workgroup, before any barrier
     uint64_t *dst = (uint64_t *)d[1];
     const uint64_t *in = in_dst ? dst + l * d[3] : (const uint64_t *)d[0] + l * d[2];
     pass_body<LOG_N, LOG_S, LOG_T, COLUMN, FWD, false>(tile, in, dst + l * d[3], tw, ninv_tab, ninv_tab, mods,
@@ -991,6 +992,7 @@ __global__ void __launch_bounds__(RB_BS)
     }
     const int64_t *d = desc + 8 * (long)lo;
     const long l = gl - limb_start[lo];
+    if (l >= d[7]) return;  // a total past the uploaded one
     const int m = (int)d[6] + (int)l;
     const uint64_t q = mods[m];
     const uint64_t a = ((const uint64_t *)d[0])[l * d[3] + k];
