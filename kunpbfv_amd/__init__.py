@@ -339,12 +339,28 @@ class Context:
                                     stream_ptr(stream), C.byref(best), C.byref(ms)))
         return best.value, ms.value, lib.kbfv_ntt_plan_bucket(limbs * polys)
 
-    def autotune(self, batch, scratch=None, reps=3, stream=None):
+    def autotune(self, batch, scratch=None, reps=3, stream=None, cache=None):
         """Tune the NTT plan of every direction and transform-count bucket at
         the sizes HomMult+relin of `batch` ciphertexts launches: over the QR
         basis, 1 polynomial (< 64 transforms), 3 (the per-ciphertext tensor),
         a mid-size batch and the whole batch's 3 tensor polynomials.  Returns
-        {(direction, bucket): (plan, ms per transform, transforms)}."""
+        {(direction, bucket): (plan, ms per transform, transforms)}.
+
+        cache: path of a JSON file of tuned plans keyed by device name, ring
+        degree, modulus table, batch and plan space; a hit sets the stored
+        plans without timing anything, a miss tunes and adds the entry."""
+        key = self._tune_key(batch) if cache else None
+        if cache:
+            hit = _cache_read(cache).get(key)
+            if hit is not None:
+                try:
+                    out = {}
+                    for d, b, plan, ms, transforms in hit:
+                        self.set_ntt_plan(d, b, plan)
+                        out[(d, b)] = (plan, ms, transforms)
+                    return out
+                except (KbfvError, TypeError, ValueError):
+                    pass  # a stale or malformed entry: tune again and replace it
         torch = _torch()
         limbs = self.qn + self.rn
         sizes = {}
@@ -362,7 +378,17 @@ class Context:
                 plan, ms, got = self.autotune_ntt(direction, self.oQ, limbs, polys, reps, scratch, stream)
                 assert got == b
                 out[(direction, b)] = (plan, ms, limbs * polys)
+        if cache:
+            _cache_write(cache, key, [[d, b, p, m, t] for (d, b), (p, m, t) in sorted(out.items())])
         return out
+
+    def _tune_key(self, batch):
+        import hashlib
+        torch = _torch()
+        mods = hashlib.sha1(np.ascontiguousarray(self.mod, dtype=np.uint64).tobytes()).hexdigest()[:16]
+        space = hashlib.sha1(np.asarray(ntt_plan_candidates(self.log_n), dtype=np.uint32).tobytes()).hexdigest()[:12]
+        dev = torch.cuda.get_device_name(torch.cuda.current_device())
+        return f"{dev}|logN={self.log_n}|mods={mods}|batch={batch}|plans={space}"
 
     # ------------------------------------------------------------ ragged batches
     def ragged(self, items, stream=None):
@@ -615,6 +641,30 @@ class Context:
         check(lib.kbfv_external_product(self.h, ptr(rgsw), ptr(ct), ptr(out), batch, base_bits, ptr(ws),
                                         ws.numel(), stream_ptr(stream)))
         return out
+
+
+def _cache_read(path):
+    import json
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _cache_write(path, key, entry):
+    """Add one entry to the JSON plan cache (written to a temporary file, then
+    renamed over the old one)."""
+    import json
+    data = _cache_read(path)
+    data[key] = entry
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    tmp = f"{path}.tmp{os.getpid()}"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
 
 
 class RaggedBatch:

@@ -105,3 +105,28 @@ def test_plan_set_rejects_invalid_words():
         ctx.set_ntt_plan(+1, 0, 0x1234567)
     with pytest.raises(kb.KbfvError):
         ctx.set_ntt_plan(+1, 9, 0)
+
+
+def test_autotune_json_cache(tmp_path):
+    """A tuned context writes its plans to the JSON cache; a second context of
+    the same parameters takes them from it without timing, a malformed entry is
+    tuned again and replaced, and a different batch is a separate entry."""
+    import json
+    kb, ctx, _ = make(14)
+    path = str(tmp_path / "plans.json")
+    first = ctx.autotune(4, reps=1, cache=path)
+    data = json.load(open(path))
+    assert len(data) == 1
+    kb2, ctx2, _ = make(14)
+    second = ctx2.autotune(4, reps=1, cache=path)
+    assert second == first
+    for (d, b), (plan, _, _) in first.items():
+        assert ctx2.ntt_plan(d, b) == plan
+    key = next(iter(data))
+    data[key][0][2] = 1 << 20  # not a plan word
+    json.dump(data, open(path, "w"))
+    third = ctx2.autotune(4, reps=1, cache=path)
+    assert all(kb.lib.kbfv_ntt_plan_valid(14, p) == 0 for p, _, _ in third.values())
+    assert json.load(open(path))[key][0][2] != 1 << 20
+    ctx2.autotune(8, reps=1, cache=path)
+    assert len(json.load(open(path))) == 2
