@@ -84,6 +84,12 @@ class BFVContext {
     void FromNTTInplace(uint64_tt *device_a, int start_poly_idx, int start_mod_idx, int poly_num, int mod_num,
                         int poly_mod_len);
 
+    // NTT plans (no counterpart in the reference, whose NTT has one shape):
+    // time every plan of both directions at the transform counts HomMult+relin
+    // of `batch` ciphertexts launches and keep the fastest per bucket
+    // (kbfv_ntt_autotune).  Call it before other work on the context.
+    void autotuneNTT(int batch);
+
     // key-stationary inner product over a batch (include/ExternalProduct.cuh:169):
     // cipher_modUp_QjtoT [batch_size][Qj_blockNum][t_num][N] (NTT over T, as
     // BFVScheme::multAndEqual_beforeIP_23 writes it) times swk_modUp_RitoT (a

@@ -169,6 +169,33 @@ void BFVContext::FromNTTInplace(uint64_tt *a, int start_poly_idx, int start_mod_
        "FromNTTInplace");
 }
 
+void BFVContext::autotuneNTT(int batch) {
+    if (batch < 1) throw std::invalid_argument("autotuneNTT: batch must be positive");
+    // over the QR basis: 1 polynomial, 3 (one tensor), a mid-size batch and the
+    // whole batch's 3 tensor polynomials, one size per transform-count bucket
+    const int limbs = q_num + r_num;
+    int polys_of[KBFV_NTT_BUCKETS] = {0};
+    for (int polys : {1, 3, std::max(1, 2048 / limbs), 3 * batch}) {
+        const int b = kbfv_ntt_plan_bucket((int64_t)limbs * polys);
+        if (b >= 0) polys_of[b] = std::max(polys_of[b], polys);
+    }
+    int most = 0;
+    for (int p : polys_of) most = std::max(most, p);
+    uint64_tt *scratch = dalloc((size_t)most * limbs * N);
+    try {
+        for (int dir : {+1, -1})
+            for (int b = 0; b < KBFV_NTT_BUCKETS; ++b)
+                if (polys_of[b])
+                    ok(kbfv_ntt_autotune(handle, dir, p_num, limbs, polys_of[b], (uint64_t *)scratch, 3, stream,
+                                         nullptr, nullptr),
+                       "autotuneNTT");
+    } catch (...) {
+        dfree(scratch);
+        throw;
+    }
+    dfree(scratch);
+}
+
 void BFVContext::external_product_T_swk_reuse(uint64_tt *output, uint64_tt *cipher_modUp_QjtoT,
                                               uint64_tt *swk_modUp_RitoT, int l, int batch_size) {
     if (l != L) throw std::invalid_argument("external_product_T_swk_reuse: ciphertexts are at full level");

@@ -21,6 +21,7 @@ static uint64_tt brv(uint64_tt x, int bits) {
 int main(int argc, char *argv[]) {
     size_t n_arg = argc > 1 ? strtoull(argv[1], nullptr, 10) : 16384;
     BFVContext context(n_arg);
+    context.autotuneNTT(2);  // everything below runs on the tuned NTT plans
     BFVScheme scheme(context);
     SecretKey sk(context);
     scheme.mallocMemory();
