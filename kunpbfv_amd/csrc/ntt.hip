@@ -7,7 +7,8 @@
 // whose entry 1 carries n^-1, outputs canonical in [0, q).
 //
 // Design (MI355X): N = N1 * N2 four-step split into two passes, each a kernel
-// that stages a 32 KiB tile in LDS and runs radix-16 rounds in registers:
+// that stages a tile of 2^11-2^13 elements (16-64 KiB, the plan's choice) in
+// LDS and runs radix-16 rounds in registers:
 //   column pass  -- C adjacent columns of stride N/N1 (coalesced C*8-byte rows),
 //                   global stages 0 .. log2(N1)-1
 //   row pass     -- C contiguous chunks of N2 elements, remaining stages
