@@ -65,3 +65,24 @@ def test_validator_rejects_malformed_words():
     assert lib.kbfv_ntt_plan_valid(11, 0) != 0                                 # ring degree
     assert [lib.kbfv_ntt_plan_bucket(n) for n in (1, 63, 64, 511, 512, 4095, 4096, 1 << 20)] == \
         [0, 0, 1, 1, 2, 2, 3, 3]
+
+
+def test_plan_cache_file_round_trip(tmp_path):
+    """The JSON plan cache (host only): entries merge into one file, a missing
+    or corrupt file reads as empty, and a write replaces the file whole."""
+    import json
+    import kunpbfv_amd as kb
+    path = str(tmp_path / "sub" / "plans.json")
+    assert kb._cache_read(path) == {}
+    kb._cache_write(path, "a", [[1, 0, 0x3cd8, 0.04, 57]])
+    kb._cache_write(path, "b", [[-1, 3, 0x1cd8, 11.2, 21888]])
+    data = kb._cache_read(path)
+    assert data == {"a": [[1, 0, 0x3cd8, 0.04, 57]], "b": [[-1, 3, 0x1cd8, 11.2, 21888]]}
+    assert json.load(open(path)) == data
+    kb._cache_write(path, "a", [[1, 0, 0, 0.05, 57]])
+    assert kb._cache_read(path)["a"] == [[1, 0, 0, 0.05, 57]]
+    open(path, "w").write("{not json")
+    assert kb._cache_read(path) == {}
+    kb._cache_write(path, "c", [])
+    assert kb._cache_read(path) == {"c": []}
+    assert not [f for f in (tmp_path / "sub").iterdir() if ".tmp" in f.name]
