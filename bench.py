@@ -24,13 +24,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# HBM bytes per k_ntt_pass launch of the roofline measurement at N=2^16,
-# 28 Q + 29 R primes, 3·128 polynomials (tuned plan: row tile 2^12, column
-# tile 2^13): rocprofv3 FETCH_SIZE and WRITE_SIZE runs over
-# `bench.py --roofline-only`, each scaled by the factor the same counter reads
-# for a 4 GiB device copy (fetch 2.000, write 1.000); recorded in
-# profiles/ntt_qr_inverse_n65536_b128_counters.md.
-NTT_TRAFFIC_BYTES = {(16, 28, 128): 23_000_583_594}
+# Measured HBM traffic of one inverse NTT over the QR basis of the whole batch
+# (the roofline entry), keyed by (log N, Q primes, batch, plan word): rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs over `bench.py --roofline-only`, summed over
+# the transform's launches and scaled by the factor each counter reads for a
+# 4 GiB device copy in the same run.  A plan without a recorded profile
+# reports traffic null rather than a figure measured for another plan.
+NTT_TRAFFIC = {
+    (16, 28, 128, 0x3cd8): (2 * 23_000_583_594, "profiles/ntt_qr_inverse_n65536_b128_counters.md"),
+}
 
 
 def parse():
@@ -455,9 +457,12 @@ def autotune_report(ctx, tuned):
 
 
 def kernel_roofline(ctx, B, stream):
-    """Average duration of one launch of the column-pass INTT kernel over the QR
-    basis for the whole batch, measured with HIP events on the stream the
-    library launches on, against its algorithmic HBM bytes."""
+    """The dominant kernel, k_ntt_pass, as the pipeline's largest NTT: one
+    inverse negacyclic NTT over the QR basis of the 3 tensor polynomials of
+    every ciphertext, timed with HIP events on the stream the library launches
+    on.  Algorithmic bytes per transform = every limb read once and written
+    once (2 x 512 KiB per 2^16-coefficient limb); a transform is two kernel
+    launches (column and row pass) per chunk of its plan."""
     import torch
     import kunpbfv_amd as kb
     sn = ctx.qn + ctx.rn
@@ -465,7 +470,6 @@ def kernel_roofline(ctx, B, stream):
     x = ctx.empty(polys, sn, ctx.n)
     x.random_(0, 1 << 40)
     reps = 5
-    # one inverse NTT = row pass + column pass; time the pair and report per launch
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ctx.ntt(x, x, -1, ctx.oQ, sn, polys=polys)
     torch.cuda.synchronize()
@@ -475,15 +479,22 @@ def kernel_roofline(ctx, B, stream):
     evs[1].record(stream)
     torch.cuda.synchronize()
     per_transform_s = evs[0].elapsed_time(evs[1]) / reps / 1e3
-    per_launch_s = per_transform_s / 2  # two passes
-    bytes_per_launch = polys * sn * ctx.n * 8 * 2  # read + write every limb once per pass
-    achieved = bytes_per_launch / per_launch_s / 1e9
-    return {"kernel": "k_ntt_pass (inverse, QR basis, whole batch)", "bound": "hbm",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": NTT_TRAFFIC_BYTES.get((ctx.log_n, ctx.qn, B)),
-            "launch_ms": round(per_launch_s * 1e3, 3), "bytes_per_launch": bytes_per_launch,
-            "ntt_plan": hex(ctx.ntt_plan(-1, kb.lib.kbfv_ntt_plan_bucket(polys * sn)))}
+    limbs = polys * sn
+    bytes_per_transform = limbs * ctx.n * 8 * 2
+    plan = ctx.ntt_plan(-1, kb.lib.kbfv_ntt_plan_bucket(limbs))
+    chunk = kb.ntt_plan_chunk_bytes(plan)
+    chunks = 1 if not chunk else -(-polys // max(1, chunk // (sn * ctx.n * 8)))
+    launches = 2 * chunks
+    achieved = bytes_per_transform / per_transform_s / 1e9
+    traffic, source = NTT_TRAFFIC.get((ctx.log_n, ctx.qn, B, plan), (None, None))
+    return {"kernel": "k_ntt_pass (inverse NTT over the QR basis, whole batch: column + row pass)",
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
+            "unit_of_work": f"one transform of {limbs} limbs of 2^{ctx.log_n} coefficients",
+            "bytes_per_transform": bytes_per_transform, "transform_ms": round(per_transform_s * 1e3, 3),
+            "launches_per_transform": launches, "avg_launch_ms": round(per_transform_s * 1e3 / launches, 4),
+            "per_pass_frac": round(achieved * 2 / HBM_PEAK_GBS, 4),
+            "ntt_plan": hex(plan), "chunk_bytes": chunk}
 
 
 if __name__ == "__main__":

@@ -55,6 +55,12 @@ int kbfv_chain_60bit(int qn, int pn, int tn, uint64_t *q, uint64_t *p, uint64_t 
  * include/uintarith_bfv.h:983); used for the R basis. */
 int kbfv_primes_below(uint64_t ntt_size, uint64_t upper, int count, uint64_t *out);
 
+/* 32 bytes from the operating system's entropy source (getrandom), for key
+ * and encryption seeds (replaces the reference's std::random_device draws,
+ * include/Utils.cuh:133-134, include/RNG.cuh:27-28).  Every seeded entry point
+ * below is deterministic in its seed; this is where fresh seeds come from. */
+int kbfv_random_seed(uint8_t out[32]);
+
 /* Default parameter set of a ring degree (replaces BFVContext::getPrimeBFV,
  * include/Params.cuh:7-148): fills at most 64 entries of q/p/t. */
 int kbfv_default_params(int log_n, uint64_t *q, int *qn, uint64_t *p, int *pn, uint64_t *t,
@@ -94,8 +100,11 @@ int kbfv_ntt(const kbfv_context *ctx, int dir, const uint64_t *src, uint64_t *ds
  * a column pass of 2^c-point sub-transforms and a row pass of 2^(logN-c)-point
  * ones.  Plan word: bits 0-3 = c, bits 4-7 / 8-11 = log2 of the column / row
  * pass tile (elements per workgroup, 11..13), bit 12 / 13 = XCD-ordered grid
- * for the column / row pass; 0 = the built-in plan.  Every plan computes the
- * same bits.  A context keeps one plan per direction and bucket of
+ * for the column / row pass, bits 14-16 = chunk k (0: each pass over the whole
+ * launch; 2..6: both passes run per chunk of at most 2^(k+23) bytes, so the
+ * second pass finds the first pass's output in the last-level cache); 0 = the
+ * built-in plan.  kbfv_ntt_plan_candidates lists the unchunked shapes; any of
+ * them with a chunk k is valid too.  Every plan computes the same bits.  A context keeps one plan per direction and bucket of
  * limbs*polys*batch (<64, <512, <4096, more); kbfv_ntt uses it. */
 #define KBFV_NTT_BUCKETS 4
 int kbfv_ntt_plan_valid(int log_n, uint32_t plan);                  /* 0 if valid; no device */
@@ -224,6 +233,13 @@ int kbfv_keygen_pk(const kbfv_context *ctx, const uint8_t seed[32], const uint64
 size_t kbfv_keygen_rlk_workspace_bytes(const kbfv_context *ctx);
 int kbfv_keygen_rlk(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *sk,
                     uint64_t *rlk, void *workspace, size_t ws_bytes, void *stream);
+/* BFVScheme::addSWKey_23 (include/KeySwitch_23.cuh:6-16, a stub there that
+ * returns an unfilled key): key-switching key from src to s, where src
+ * [qn][N] is any polynomial over Q in NTT form (src = s^2 gives a
+ * relinearisation key, sampled under other stream tags than
+ * kbfv_keygen_rlk's).  Layout and workspace as kbfv_keygen_rlk. */
+int kbfv_keygen_swk(const kbfv_context *ctx, const uint8_t seed[32], const uint64_t *sk,
+                    const uint64_t *src_q, uint64_t *key, void *workspace, size_t ws_bytes, void *stream);
 /* BFVScheme::encryptMsg (include/BFVScheme.cuh:118-146); entry b of the batch
  * draws its randomness with tag index b.  m [batch][N] encoded plaintexts. */
 size_t kbfv_encrypt_workspace_bytes(const kbfv_context *ctx, int batch);
@@ -246,6 +262,14 @@ size_t kbfv_mult_workspace_bytes(const kbfv_context *ctx, int batch);
 int kbfv_mult_relin(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t *ct1,
                     const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
                     size_t ws_bytes, void *stream);
+
+/* BFVScheme::compute_t_QInv_c0c1c2 (include/KeySwitch_23.cuh:703-926), batched:
+ * the HPS-scaled tensor of each pair, out3 [batch][3][qn][N] = (a1a2 = axax_mul,
+ * a1b2 + a2b1 = axbx1_mul, b1b2 = bxbx_mul), NTT form over Q; without the
+ * reference's secret-key shortcut (:823-832).  Workspace:
+ * kbfv_mult_workspace_bytes. */
+int kbfv_mult_tensor(const kbfv_context *ctx, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out3,
+                     int batch, void *workspace, size_t ws_bytes, void *stream);
 
 /* The reference's split-IP seam, batched (BFVScheme::multAndEqual_beforeIP_23 /
  * multAndEqual_afterIP_23, include/KeySwitch_23.cuh:312-372).  before_ip runs

@@ -133,7 +133,9 @@ class SecretKey {
    public:
     uint64_tt *sx_device = nullptr;
     int N = 0, L = 0, K = 0;
+    // a fresh key from OS entropy (the reference's SecretKey(context))
     explicit SecretKey(BFVContext &context, hipStream_t stream = 0);
+    // a reproducible key from an explicit seed
     SecretKey(BFVContext &context, const uint8_t seed[32], hipStream_t stream = 0);
     ~SecretKey();
     SecretKey(const SecretKey &) = delete;
@@ -169,7 +171,9 @@ class BFVScheme {
 
     // scratch for single-ciphertext operations (grows on demand for batches)
     void mallocMemory();
-    // deterministic randomness: every key / encryption draws its seed from here
+    // Every key / encryption draws its seed from a generator that starts from
+    // OS entropy; setSeed restarts it from `seed`, which makes keys and
+    // encryptions reproducible (tests, benchmarks).
     void setSeed(uint64_t seed);
 
     void addEncKey(SecretKey &secretKey, hipStream_t stream = 0);
@@ -177,7 +181,14 @@ class BFVScheme {
     // Galois keys: left rotation by rot (g = 5^rot mod 2N) and conjugation (g = 2N - 1)
     void addLeftRotKey_23(SecretKey &secretKey, long rot, hipStream_t stream = 0);
     void addConjKey_23(SecretKey &secretKey, hipStream_t stream = 0);
+    // automorphism X -> X^(2^d + 1) (include/KeySwitch_23.cuh:171-190, a stub there)
+    void addAutoKey_23(SecretKey &secretKey, int d, hipStream_t stream = 0);
+    // key-switching key from s2 (q_num limbs over Q, NTT form) to the secret key
+    // (include/KeySwitch_23.cuh:6-16, a stub there); the caller owns the result
+    Key_decomp *addSWKey_23(SecretKey &secretKey, uint64_tt *s2, hipStream_t stream = 0);
 
+    // encryption of zero (include/BFVScheme.cuh:87-116); l must be the full level L
+    void encryptZero(Ciphertext &cipher, int l, int slots, hipStream_t stream = 0);
     void encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t stream = 0);
     void decryptMsg(Plaintext &plain, SecretKey &secretKey, Ciphertext &cipher, hipStream_t stream = 0);
 
@@ -217,20 +228,35 @@ class BFVScheme {
     void rightRotateAndEqual_23(Ciphertext &cipher, long rotSlots);  // left by N/2 - 2^rotSlots
     void conjugate_23(Ciphertext &cipher_res, Ciphertext &cipher);
     void conjugateAndEqual_23(Ciphertext &cipher);
+    // f(X) -> f(X) + f(X^(2^d + 1)) (include/KeySwitch_23.cuh:587-589, empty there)
+    void automorphismAndAdd(Ciphertext &cipher, int d);
+
+    // HPS decryption rounding of b + a*s in coefficient form (q_num limbs) into
+    // dst [N] (include/BFVScheme.cuh:358-398)
+    void hps_decrypt_scale_and_round(uint64_tt *dst, uint64_tt *src, const hipStream_t &stream) const;
+    // the t/Q-scaled tensor of HomMult, each [q_num][N] in NTT form:
+    // axbx1 = a1b2 + a2b1, axa = a1a2, bxbx = b1b2 (include/KeySwitch_23.cuh:703-926)
+    void compute_t_QInv_c0c1c2(uint64_tt *axbx1, uint64_tt *axa, uint64_tt *bxbx, Ciphertext &cipher1,
+                               Ciphertext &cipher2);
 
     BFVContext &context;
     Key *publicKey = nullptr;
     Key_decomp *rlk_23 = nullptr;
-    std::map<long, Key_decomp *> rotKey_vec_23;  // by rotation amount
+    std::vector<Key_decomp *> rotKey_vec_23;   // [N], by rotation amount (null: not generated)
     Key_decomp *ConjKey_23 = nullptr;
+    std::vector<Key_decomp *> autoKey_vec_23;  // [logN], by d of X -> X^(2^d + 1)
 
    private:
     void seed_next(uint8_t out[32]);
     Key_decomp *galois_key(SecretKey &sk, uint64_t g, hipStream_t stream);
     void apply_galois(Ciphertext &res, Ciphertext &cipher, Key_decomp *key, uint64_t g);
     void *ensure_ws(size_t bytes);
-    uint64_t seed_state = 0x6b756e7042465600ULL;
+    uint64_t seed_state = 0;  // set from OS entropy by the constructor
     void *ws = nullptr;
     size_t ws_bytes = 0;
     uint64_tt *d01 = nullptr;  // [2][q_num][N] linear terms of the split-IP seam
+    // gather buffers of multAndEqual_23_batch for callers whose ciphertexts are
+    // not one contiguous array; grown once to the largest batch seen
+    uint64_tt *batch_a = nullptr, *batch_b = nullptr;
+    size_t batch_cap = 0;
 };

@@ -25,6 +25,7 @@ SIGNATURES = {
     "kbfv_last_error": (C.c_char_p, []),
     "kbfv_chain_60bit": (C.c_int, [C.c_int, C.c_int, C.c_int, _u64p, _u64p, _u64p]),
     "kbfv_primes_below": (C.c_int, [_u64, _u64, C.c_int, _u64p]),
+    "kbfv_random_seed": (C.c_int, [_vp]),
     "kbfv_default_params": (C.c_int, [C.c_int, _u64p, _ip, _u64p, _ip, _u64p, _ip, _ip, _u64p]),
     "kbfv_context_create": (C.c_int, [C.c_int, _u64p, C.c_int, _u64p, C.c_int, _u64p, C.c_int, C.c_int,
                                       _u64, C.POINTER(_vp)]),
@@ -65,6 +66,8 @@ SIGNATURES = {
     "kbfv_keygen_pk": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "kbfv_keygen_rlk_workspace_bytes": (_sz, [_vp]),
     "kbfv_keygen_rlk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "kbfv_keygen_swk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "kbfv_mult_tensor": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
     "kbfv_encrypt_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_encrypt": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
     "kbfv_decrypt_workspace_bytes": (_sz, [_vp, C.c_int]),
@@ -154,10 +157,17 @@ def ntt_plan_candidates(log_n):
 
 def ntt_plan_fields(plan, log_n):
     """Decode a plan word: (column-pass log length, column tile log, row tile log,
-    column XCD order, row XCD order); 0 is the built-in plan."""
+    column XCD order, row XCD order); 0 is the built-in plan.  The chunk field
+    is ntt_plan_chunk_bytes."""
     if plan == 0:
         return ((log_n + 1) // 2, 12, 12, 1, 1)
     return (plan & 15, (plan >> 4) & 15, (plan >> 8) & 15, (plan >> 12) & 1, (plan >> 13) & 1)
+
+
+def ntt_plan_chunk_bytes(plan):
+    """Bytes per chunk of a chunked plan (both passes per chunk), 0 if unchunked."""
+    k = (plan >> 14) & 7
+    return 1 << (k + 23) if k else 0
 
 
 def chain_60bit(qn, pn, tn):
@@ -224,6 +234,13 @@ def wire_checksum(words):
 def to_host(t):
     """torch.int64 device tensor -> numpy uint64 (same bits)"""
     return t.detach().cpu().numpy().view(np.uint64)
+
+
+def random_seed():
+    """32 fresh bytes from the OS entropy source (kbfv_random_seed)."""
+    buf = C.create_string_buffer(32)
+    check(lib.kbfv_random_seed(buf))
+    return buf.raw
 
 
 def _seed(seed):
@@ -388,7 +405,7 @@ class Context:
         mods = hashlib.sha1(np.ascontiguousarray(self.mod, dtype=np.uint64).tobytes()).hexdigest()[:16]
         space = hashlib.sha1(np.asarray(ntt_plan_candidates(self.log_n), dtype=np.uint32).tobytes()).hexdigest()[:12]
         dev = torch.cuda.get_device_name(torch.cuda.current_device())
-        return f"{dev}|logN={self.log_n}|mods={mods}|batch={batch}|plans={space}"
+        return f"{dev}|logN={self.log_n}|mods={mods}|batch={batch}|plans={space}|build={library_build_id()}"
 
     # ------------------------------------------------------------ ragged batches
     def ragged(self, items, stream=None):
@@ -483,6 +500,14 @@ class Context:
                                   stream_ptr(stream)))
         return rlk
 
+    def keygen_swk(self, seed, sk, src_q, stream=None):
+        """key-switching key from src_q ([qn][N], NTT over Q) to the secret"""
+        key = self.empty(2, self.dnum, self.ri_blocks, self.tn, self.n)
+        ws = self.workspace(lib.kbfv_keygen_rlk_workspace_bytes(self.h))
+        check(lib.kbfv_keygen_swk(self.h, _seed(seed), ptr(sk), ptr(src_q), ptr(key), ptr(ws), ws.numel(),
+                                  stream_ptr(stream)))
+        return key
+
     def encrypt(self, seed, pk, m, stream=None):
         """m: [batch][N] encoded plaintexts -> [batch][2][qn][N]"""
         batch = m.shape[0]
@@ -514,6 +539,15 @@ class Context:
         ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
         check(lib.kbfv_mult_relin(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
                                   stream_ptr(stream)))
+        return out
+
+    def mult_tensor(self, c1, c2, ws=None, stream=None):
+        """the scaled tensor (a1a2, a1b2+a2b1, b1b2) of each pair, NTT form: [batch][3][qn][N]"""
+        batch = c1.shape[0]
+        out = self.empty(batch, 3, self.qn, self.n)
+        ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(batch))
+        check(lib.kbfv_mult_tensor(self.h, ptr(c1), ptr(c2), ptr(out), batch, ptr(ws), ws.numel(),
+                                   stream_ptr(stream)))
         return out
 
     def mult_before_ip(self, c1, c2, ws=None, stream=None):
@@ -653,18 +687,42 @@ def _cache_read(path):
         return {}
 
 
+_BUILD_ID = None
+
+
+def library_build_id():
+    """Hash of the loaded libkunpbfv.so: tuned plans are kept per build, so a
+    changed kernel is timed again rather than trusted from an older cache."""
+    global _BUILD_ID
+    if _BUILD_ID is None:
+        import hashlib
+        h = hashlib.sha1()
+        with open(LIB_PATH, "rb") as f:
+            for block in iter(lambda: f.read(1 << 20), b""):
+                h.update(block)
+        _BUILD_ID = h.hexdigest()[:16]
+    return _BUILD_ID
+
+
 def _cache_write(path, key, entry):
-    """Add one entry to the JSON plan cache (written to a temporary file, then
-    renamed over the old one)."""
+    """Add one entry to the JSON plan cache: read, update and rename a
+    temporary file over the old one while holding an exclusive lock on a
+    side file, so processes tuning at once keep each other's entries."""
+    import fcntl
     import json
-    data = _cache_read(path)
-    data[key] = entry
     d = os.path.dirname(os.path.abspath(path))
     os.makedirs(d, exist_ok=True)
-    tmp = f"{path}.tmp{os.getpid()}"
-    with open(tmp, "w") as f:
-        json.dump(data, f, indent=1, sort_keys=True)
-    os.replace(tmp, path)
+    with open(f"{path}.lock", "a") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        try:
+            data = _cache_read(path)
+            data[key] = entry
+            tmp = f"{path}.tmp{os.getpid()}"
+            with open(tmp, "w") as f:
+                json.dump(data, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+        finally:
+            fcntl.flock(lk, fcntl.LOCK_UN)
 
 
 class RaggedBatch:

@@ -215,12 +215,8 @@ static uint64_t splitmix(uint64_t &s) {
 }
 
 SecretKey::SecretKey(BFVContext &context, hipStream_t stream) : N((int)context.N), L(context.L), K(context.K) {
-    uint64_t s = 0x5345435245544b59ULL;  // fixed default: keys are reproducible run to run
-    uint8_t seed[32];
-    for (int i = 0; i < 4; ++i) {
-        uint64_t v = splitmix(s);
-        memcpy(seed + 8 * i, &v, 8);
-    }
+    uint8_t seed[32];  // fresh from the OS, as the reference's std::random_device (include/Utils.cuh:133)
+    ok(kbfv_random_seed(seed), "secret key seed");
     sx_device = dalloc((size_t)N * (K + L + 1));
     ok(kbfv_keygen_sk(context.handle, seed, (uint64_t *)sx_device, stream), "secret key");
     hip_ok(hipStreamSynchronize(stream), "secret key");
@@ -250,14 +246,24 @@ Key_decomp::Key_decomp(int N_, int dnum_, int t_num_, int blocks) : N(N_), dnum(
 Key_decomp::~Key_decomp() { dfree(cipher_device); }
 
 // ---------------------------------------------------------------- BFVScheme
-BFVScheme::BFVScheme(BFVContext &ctx) : context(ctx) {}
+BFVScheme::BFVScheme(BFVContext &ctx)
+    : context(ctx), rotKey_vec_23((size_t)ctx.N, nullptr), autoKey_vec_23((size_t)ctx.logN, nullptr) {
+    // fresh randomness per scheme (the reference re-seeds from std::random_device,
+    // include/RNG.cuh:27-28); setSeed makes keys and encryptions reproducible
+    uint8_t s[32];
+    ok(kbfv_random_seed(s), "scheme seed");
+    memcpy(&seed_state, s, sizeof(seed_state));
+}
 BFVScheme::~BFVScheme() {
     delete publicKey;
     delete rlk_23;
-    for (auto &kv : rotKey_vec_23) delete kv.second;
+    for (auto *k : rotKey_vec_23) delete k;
+    for (auto *k : autoKey_vec_23) delete k;
     delete ConjKey_23;
     dfree(ws);
     dfree(d01);
+    dfree(batch_a);
+    dfree(batch_b);
 }
 
 void BFVScheme::setSeed(uint64_t seed) { seed_state = seed; }
@@ -322,8 +328,43 @@ Key_decomp *BFVScheme::galois_key(SecretKey &sk, uint64_t g, hipStream_t stream)
 }
 
 void BFVScheme::addLeftRotKey_23(SecretKey &sk, long rot, hipStream_t stream) {
-    if (rotKey_vec_23.count(rot)) return;  // as the reference: generated once
+    if (rot < 0 || rot >= (long)rotKey_vec_23.size()) throw std::invalid_argument("rotation out of range");
+    if (rotKey_vec_23[rot]) return;  // as the reference: generated once
     rotKey_vec_23[rot] = galois_key(sk, kbfv_galois_elt(context.handle, rot), stream);
+}
+
+static uint64_t auto_elt(const BFVContext &c, int d) { return ((1ULL << d) + 1) % (2 * (uint64_t)c.N); }
+
+void BFVScheme::addAutoKey_23(SecretKey &sk, int d, hipStream_t stream) {
+    if (d < 0 || d >= (int)autoKey_vec_23.size()) throw std::invalid_argument("autoKey only for d in [0, logN)");
+    if (autoKey_vec_23[d]) return;
+    autoKey_vec_23[d] = galois_key(sk, auto_elt(context, d), stream);
+}
+
+void BFVScheme::automorphismAndAdd(Ciphertext &cipher, int d) {
+    if (d < 0 || d >= (int)autoKey_vec_23.size() || !autoKey_vec_23[d])
+        throw std::invalid_argument("autoKey_23 not exists");
+    Ciphertext tmp(cipher.N, cipher.L, cipher.l, cipher.slots);
+    apply_galois(tmp, cipher, autoKey_vec_23[d], auto_elt(context, d));
+    add(cipher, cipher, tmp);
+    hip_ok(hipStreamSynchronize(context.stream), "automorphismAndAdd");
+}
+
+Key_decomp *BFVScheme::addSWKey_23(SecretKey &sk, uint64_tt *s2, hipStream_t stream) {
+    auto *key = new Key_decomp((int)context.N, context.dnum, context.t_num, context.Ri_blockNum);
+    uint8_t seed[32];
+    seed_next(seed);
+    try {
+        void *w = ensure_ws(kbfv_keygen_rlk_workspace_bytes(context.handle));
+        ok(kbfv_keygen_swk(context.handle, seed, (uint64_t *)sk.sx_device, (const uint64_t *)s2,
+                           (uint64_t *)key->cipher_device, w, ws_bytes, stream),
+           "switching key");
+        hip_ok(hipStreamSynchronize(stream), "switching key");
+    } catch (...) {
+        delete key;
+        throw;
+    }
+    return key;
 }
 
 void BFVScheme::addConjKey_23(SecretKey &sk, hipStream_t stream) {
@@ -342,9 +383,9 @@ void BFVScheme::apply_galois(Ciphertext &res, Ciphertext &cipher, Key_decomp *ke
 }
 
 void BFVScheme::leftRotateAndEqual_23(Ciphertext &cipher, long rotSlots) {
-    auto it = rotKey_vec_23.find(rotSlots);
-    if (it == rotKey_vec_23.end()) throw std::invalid_argument("rotKey_23 not exists");
-    apply_galois(cipher, cipher, it->second, kbfv_galois_elt(context.handle, rotSlots));
+    if (rotSlots < 0 || rotSlots >= (long)rotKey_vec_23.size() || !rotKey_vec_23[rotSlots])
+        throw std::invalid_argument("rotKey_23 not exists");
+    apply_galois(cipher, cipher, rotKey_vec_23[rotSlots], kbfv_galois_elt(context.handle, rotSlots));
 }
 
 void BFVScheme::rightRotateAndEqual_23(Ciphertext &cipher, long rotSlots) {
@@ -369,6 +410,46 @@ void BFVScheme::encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t s
                     (uint64_t *)cipher.cipher_device, 1, w, ws_bytes, stream),
        "encrypt");
     hip_ok(hipStreamSynchronize(stream), "encrypt");
+}
+
+void BFVScheme::encryptZero(Ciphertext &cipher, int l, int slots, hipStream_t stream) {
+    if (!publicKey) throw std::invalid_argument("public key not generated (call addEncKey)");
+    if (l != context.L) throw std::invalid_argument("encryptZero: this engine encrypts at the full level L");
+    cipher.l = l;
+    cipher.slots = slots;
+    const size_t enc = kbfv_encrypt_workspace_bytes(context.handle, 1);
+    // the zero message sits in the workspace, behind the encryption's own part
+    uint8_t *w = (uint8_t *)ensure_ws(enc + 256 + (size_t)context.N * 8);
+    uint64_t *zero = (uint64_t *)(w + ((enc + 255) & ~size_t(255)));
+    hip_ok(hipMemsetAsync(zero, 0, (size_t)context.N * 8, stream), "encryptZero");
+    uint8_t seed[32];
+    seed_next(seed);
+    ok(kbfv_encrypt(context.handle, seed, (uint64_t *)publicKey->cipher_device, zero,
+                    (uint64_t *)cipher.cipher_device, 1, w, enc, stream),
+       "encryptZero");
+    hip_ok(hipStreamSynchronize(stream), "encryptZero");
+}
+
+void BFVScheme::hps_decrypt_scale_and_round(uint64_tt *dst, uint64_tt *src, const hipStream_t &stream) const {
+    ok(kbfv_hps_decrypt(context.handle, (const uint64_t *)src, (uint64_t *)dst, 1, stream),
+       "hps_decrypt_scale_and_round");
+}
+
+void BFVScheme::compute_t_QInv_c0c1c2(uint64_tt *axbx1, uint64_tt *axa, uint64_tt *bxbx, Ciphertext &c1,
+                                      Ciphertext &c2) {
+    if (c1.l == 0 || c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    c1.l = std::min(c1.l, c2.l);
+    const size_t poly = (size_t)context.N * context.q_num;
+    const size_t mws = kbfv_mult_workspace_bytes(context.handle, 1);
+    uint8_t *w = (uint8_t *)ensure_ws(mws + 256 + 3 * poly * 8);
+    uint64_t *out3 = (uint64_t *)(w + ((mws + 255) & ~size_t(255)));
+    ok(kbfv_mult_tensor(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device, out3, 1, w,
+                        mws, context.stream),
+       "compute_t_QInv_c0c1c2");
+    hip_ok(hipMemcpyAsync(axa, out3, poly * 8, hipMemcpyDeviceToDevice, context.stream), "tensor copy");
+    hip_ok(hipMemcpyAsync(axbx1, out3 + poly, poly * 8, hipMemcpyDeviceToDevice, context.stream), "tensor copy");
+    hip_ok(hipMemcpyAsync(bxbx, out3 + 2 * poly, poly * 8, hipMemcpyDeviceToDevice, context.stream), "tensor copy");
+    hip_ok(hipStreamSynchronize(context.stream), "compute_t_QInv_c0c1c2");
 }
 
 void BFVScheme::decryptMsg(Plaintext &plain, SecretKey &sk, Ciphertext &cipher, hipStream_t stream) {
@@ -481,28 +562,63 @@ void BFVScheme::multAndEqual_afterIP_23(Ciphertext &c1, Ciphertext &c2, uint64_t
     hip_ok(hipStreamSynchronize(context.stream), "multAndEqual_afterIP_23");
 }
 
+// All pairs in one batched pipeline (the relinearisation key is read once per
+// chunk).  Ciphertexts that already form one contiguous array (c[i] at
+// c[0] + i * 2 * N * (L+1) words) are used in place; otherwise they are
+// gathered into scheme-owned buffers that grow once to the largest batch and
+// are reused, so no call after the first allocates device memory.
 void BFVScheme::multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector<Ciphertext *> &c2) {
     if (c1.size() != c2.size()) throw std::invalid_argument("batch sizes differ");
     if (!rlk_23) throw std::invalid_argument("rotKey_23 not exists");
     const int B = (int)c1.size();
     if (B == 0) return;
-    const size_t ct = (size_t)2 * context.N * (context.L + 1);
-    // gather into contiguous batches, run one batched call, scatter back
-    uint64_tt *a = dalloc(ct * B), *b = dalloc(ct * B);
-    for (int i = 0; i < B; ++i) {
+    for (int i = 0; i < B; ++i) {  // every check before any device work
+        if (!c1[i] || !c2[i]) throw std::invalid_argument("null ciphertext");
         if (c1[i]->l == 0 || c2[i]->l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
-        hip_ok(hipMemcpyAsync(a + i * ct, c1[i]->cipher_device, ct * 8, hipMemcpyDeviceToDevice, context.stream), "gather");
-        hip_ok(hipMemcpyAsync(b + i * ct, c2[i]->cipher_device, ct * 8, hipMemcpyDeviceToDevice, context.stream), "gather");
+        if (c1[i]->N != (int)context.N || c1[i]->L != context.L || c2[i]->N != (int)context.N ||
+            c2[i]->L != context.L)
+            throw std::invalid_argument("ciphertext shape differs from the context");
     }
-    void *w = ensure_ws(kbfv_mult_workspace_bytes(context.handle, B));
+    const size_t ct = (size_t)2 * context.N * (context.L + 1);
+    auto contiguous = [&](std::vector<Ciphertext *> &v) {
+        for (int i = 1; i < B; ++i)
+            if (v[i]->cipher_device != v[0]->cipher_device + i * ct) return false;
+        return true;
+    };
+    const bool in1 = contiguous(c1), in2 = contiguous(c2);
+    if ((!in1 || !in2) && batch_cap < (size_t)B) {
+        dfree(batch_a);
+        dfree(batch_b);
+        batch_a = batch_b = nullptr;
+        batch_cap = 0;
+        batch_a = dalloc(ct * B);
+        batch_b = dalloc(ct * B);
+        batch_cap = (size_t)B;
+    }
+    hipStream_t st = context.stream;
+    uint64_tt *a = in1 ? c1[0]->cipher_device : batch_a, *b = in2 ? c2[0]->cipher_device : batch_b;
+    for (int i = 0; i < B; ++i) {
+        if (!in1) hip_ok(hipMemcpyAsync(a + i * ct, c1[i]->cipher_device, ct * 8, hipMemcpyDeviceToDevice, st), "gather");
+        if (!in2) hip_ok(hipMemcpyAsync(b + i * ct, c2[i]->cipher_device, ct * 8, hipMemcpyDeviceToDevice, st), "gather");
+    }
+    // the workspace sized for the whole batch when it fits in 1/4 of free device
+    // memory, else for chunks (kbfv_mult_relin chunks the batch to fit)
+    size_t want = kbfv_mult_workspace_bytes(context.handle, B);
+    if (want > ws_bytes) {
+        size_t free_b = 0, total_b = 0;
+        hip_ok(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const size_t one = kbfv_mult_workspace_bytes(context.handle, 1);
+        want = std::max(one, std::min(want, free_b / 4));
+    }
+    void *w = ensure_ws(want);
     ok(kbfv_mult_relin(context.handle, (uint64_t *)rlk_23->cipher_device, (uint64_t *)a, (uint64_t *)b,
-                       (uint64_t *)a, B, w, ws_bytes, context.stream),
+                       (uint64_t *)a, B, w, ws_bytes, st),
        "batched mult");
-    for (int i = 0; i < B; ++i)
-        hip_ok(hipMemcpyAsync(c1[i]->cipher_device, a + i * ct, ct * 8, hipMemcpyDeviceToDevice, context.stream), "scatter");
-    hip_ok(hipStreamSynchronize(context.stream), "batched mult");
-    dfree(a);
-    dfree(b);
+    for (int i = 0; i < B; ++i) {
+        c1[i]->l = std::min(c1[i]->l, c2[i]->l);
+        if (!in1) hip_ok(hipMemcpyAsync(c1[i]->cipher_device, a + i * ct, ct * 8, hipMemcpyDeviceToDevice, st), "scatter");
+    }
+    hip_ok(hipStreamSynchronize(st), "batched mult");
 }
 
 void BFVScheme::multConstAndEqual(Ciphertext &cipher, uint64_tt cnst) {

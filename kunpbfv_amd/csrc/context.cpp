@@ -6,6 +6,9 @@
 // include/Params.cuh:7-148) and BaseConverter::init
 // (include/BasisConv_QR.cuh:53-189).  Host arithmetic uses unsigned __int128
 // and a small multi-word integer where products of many primes are needed.
+#include <errno.h>
+#include <sys/random.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -171,6 +174,25 @@ uint64_t prod_except(const std::vector<uint64_t> &m, int lo, int hi, int skip, u
 }  // namespace
 
 // ---------------------------------------------------------------- parameter API
+// 32 bytes from the operating system's entropy source (getrandom(2), which
+// blocks only until the kernel's pool is first initialised).  The reference
+// draws its key and encryption randomness from std::random_device
+// (include/Utils.cuh:133-134, include/RNG.cuh:27-28).
+extern "C" int kbfv_random_seed(uint8_t out[32]) {
+    KB_REQUIRE(out, "null argument");
+    size_t got = 0;
+    while (got < 32) {
+        const ssize_t r = getrandom(out + got, 32 - got, 0);
+        if (r < 0) {
+            if (errno == EINTR) continue;
+            kb_set_error("getrandom failed: %s", strerror(errno));
+            return KBFV_EINVAL;
+        }
+        got += (size_t)r;
+    }
+    return KBFV_OK;
+}
+
 extern "C" int kbfv_primes_below(uint64_t ntt_size, uint64_t upper, int count, uint64_t *out) {
     // include/uintarith_bfv.h:983 get_primes_below
     uint64_t factor = 2 * ntt_size, value = upper - factor;
@@ -320,6 +342,12 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
             if (mod[j] == mod[i]) return fail("RNS moduli must be distinct");
     }
     c->batching = ((plain - 1) % (2 * (uint64_t)N)) == 0 && is_prime(plain);
+    {
+        uint64_t qmax = 0, tmin = ~0ULL;
+        for (int i = 0; i < qn; i++) qmax = std::max(qmax, q[i]);
+        for (int i = 0; i < tn; i++) tmin = std::min(tmin, t[i]);
+        c->digits_direct_t = pn == 1 && qmax / 8 < tmin;
+    }
 
     // twiddles: bit-reversed psi powers, n^-1 folded into psiinv[1]
     // (include/Utils.cuh:171-191)

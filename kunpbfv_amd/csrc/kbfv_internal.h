@@ -108,6 +108,10 @@ struct kbfv_context {
     int qMSB = 0, sizeQMSB = 0, tMSB = 0;
     uint64_t negQ_mod_t = 0, negQ_mod_t_shoup = 0;
     int device = 0;
+    // K = 1 and every Q prime below 8 * min(T): a Q limb is a valid lazy input
+    // of the forward NTT over every T prime, so the digit -> T conversion is
+    // that NTT reading the Q limb (kb_digits_to_t_ntt)
+    bool digits_direct_t = false;
 
     // host copies (exported to parity tests)
     std::vector<uint64_t> mod, psi;

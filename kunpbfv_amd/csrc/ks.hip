@@ -467,9 +467,10 @@ int kb_digits_to_t_ntt(const kbfv_context *c, const uint64_t *x_coeff, long x_st
                        int batch, uint64_t *tmp, hipStream_t st) {
     const long N = c->n;
     const int D = c->dnum, tn = c->tn;
-    if (c->pn == 1) {
-        // digit d is Q limb d itself: NTT over each T prime straight from the Q limb
-        // (x < q_d < 2^60 is a valid lazy input for every T prime)
+    if (c->digits_direct_t) {
+        // digit d is Q limb d itself: NTT over each T prime straight from the Q
+        // limb (x < q_d < 8 t_j is a valid lazy input of the forward butterfly);
+        // other chains reduce the digit into T first (k_modup_qj_to_t)
         return kb_ntt(c, +1, x_coeff, out, c->oT, tn, D, batch, x_stride, N, 0, (long)D * tn * N,
                       (long)tn * N, N, st);
     }

@@ -586,11 +586,13 @@ __global__ void __launch_bounds__((1 << LOG_T) >> LOG_R)
 // ---------------------------------------------------------------- plans
 struct plan_fields {
     int col_s, col_t, row_t, col_x, row_x;
+    int chunk;  // 0: each pass over the whole launch; k: chunks of at most 2^(k+23) bytes
 };
 
 plan_fields decode_plan(int log_n, uint32_t p) {
-    if (p == 0) return {col_log_default(log_n), 12, 12, 1, 1};
-    return {(int)(p & 15), (int)((p >> 4) & 15), (int)((p >> 8) & 15), (int)((p >> 12) & 1), (int)((p >> 13) & 1)};
+    if (p == 0) return {col_log_default(log_n), 12, 12, 1, 1, 0};
+    return {(int)(p & 15),         (int)((p >> 4) & 15), (int)((p >> 8) & 15),
+            (int)((p >> 12) & 1),  (int)((p >> 13) & 1), (int)((p >> 14) & 7)};
 }
 
 // single-pass plan: column length = log N, no tiles, row order bit clear
@@ -598,17 +600,19 @@ bool single_pass(int log_n, const plan_fields &f) { return f.col_s == log_n; }
 
 bool plan_ok(int log_n, uint32_t p) {
     if (p == 0) return true;
-    if (p >> 14) return false;
+    if (p >> 17) return false;
     const plan_fields f = decode_plan(log_n, p);
-    if (single_pass(log_n, f)) return log_n <= 14 && f.col_t == 0 && f.row_t == 0 && f.row_x == 0;
+    if (single_pass(log_n, f))
+        return log_n <= 14 && f.col_t == 0 && f.row_t == 0 && f.row_x == 0 && f.chunk == 0;
+    if (f.chunk != 0 && (f.chunk < 2 || f.chunk > 6)) return false;
     const int d = f.col_s - col_log_default(log_n);
     return d >= -1 && d <= 1 && pass_ok(log_n, f.col_s, f.col_t, true) &&
            pass_ok(log_n, log_n - f.col_s, f.row_t, false);
 }
 
-uint32_t encode_plan(int col_s, int col_t, int row_t, int col_x, int row_x) {
+uint32_t encode_plan(int col_s, int col_t, int row_t, int col_x, int row_x, int chunk = 0) {
     return (uint32_t)col_s | (uint32_t)col_t << 4 | (uint32_t)row_t << 8 | (uint32_t)col_x << 12 |
-           (uint32_t)row_x << 13;
+           (uint32_t)row_x << 13 | (uint32_t)chunk << 14;
 }
 
 int bucket_of(long transforms) {
@@ -662,18 +666,46 @@ int ntt_with_plan(const kbfv_context *c, int dir, bool noscale, uint32_t plan, c
         return noscale ? launch_limb_n<false, true>(a) : launch_limb_n<false, false>(a);
     }
     const int row_s = c->log_n - f.col_s;
-    pass_args first{c, src, dst, mod0, limbs, polys, batch, sB, sY, sL, dB, dY, dL, dir > 0 ? f.col_x : f.row_x, st};
-    pass_args second{c, dst, dst, mod0, limbs, polys, batch, dB, dY, dL, dB, dY, dL, dir > 0 ? f.row_x : f.col_x, st};
-    int rc;
-    if (dir > 0) {
-        rc = launch_pass_n<true, true>(f.col_s, f.col_t, first);
-        if (rc) return rc;
-        return launch_pass_n<false, true>(row_s, f.row_t, second);
+    // both passes of one chunk of the launch before the next chunk: a chunk
+    // small enough for the 256 MB last-level (Infinity) cache is still there
+    // when its second pass reads what its first pass wrote.  The chunk runs
+    // over the outermost dimension with more than one entry (batch, then
+    // polynomials, then limbs).
+    const long limb_bytes = (long)c->n * 8;
+    const long cap = f.chunk ? (1L << (f.chunk + 23)) : 0;
+    int dim = 0, total = 1, step = 1;
+    if (cap) {
+        if (batch > 1)
+            dim = 0, total = batch, step = (int)std::max(1L, cap / (limb_bytes * limbs * polys));
+        else if (polys > 1)
+            dim = 1, total = polys, step = (int)std::max(1L, cap / (limb_bytes * limbs));
+        else
+            dim = 2, total = limbs, step = (int)std::max(1L, cap / limb_bytes);
+        if (step >= total) step = total;
     }
-    rc = launch_pass_n<false, false>(row_s, f.row_t, first);
-    if (rc) return rc;
-    if (noscale) return launch_pass_n<true, false, true>(f.col_s, f.col_t, second);
-    return launch_pass_n<true, false>(f.col_s, f.col_t, second);
+    for (int k0 = 0; k0 < total; k0 += step) {
+        const int kn = std::min(step, total - k0);
+        int cb = batch, cp = polys, cl = limbs, cm = mod0;
+        long so = 0, dof = 0;
+        if (cap && dim == 0) cb = kn, so = k0 * sB, dof = k0 * dB;
+        if (cap && dim == 1) cp = kn, so = k0 * sY, dof = k0 * dY;
+        if (cap && dim == 2) cl = kn, cm = mod0 + k0, so = k0 * sL, dof = k0 * dL;
+        pass_args first{c, src + so, dst + dof, cm, cl, cp, cb, sB, sY, sL, dB, dY, dL,
+                        dir > 0 ? f.col_x : f.row_x, st};
+        pass_args second{c, dst + dof, dst + dof, cm, cl, cp, cb, dB, dY, dL, dB, dY, dL,
+                         dir > 0 ? f.row_x : f.col_x, st};
+        if (dir > 0) {
+            KB_TRY((launch_pass_n<true, true>(f.col_s, f.col_t, first)));
+            KB_TRY((launch_pass_n<false, true>(row_s, f.row_t, second)));
+            continue;
+        }
+        KB_TRY((launch_pass_n<false, false>(row_s, f.row_t, first)));
+        if (noscale)
+            KB_TRY((launch_pass_n<true, false, true>(f.col_s, f.col_t, second)));
+        else
+            KB_TRY((launch_pass_n<true, false>(f.col_s, f.col_t, second)));
+    }
+    return KBFV_OK;
 }
 
 }  // namespace
@@ -885,6 +917,38 @@ extern "C" int kbfv_ntt_autotune(kbfv_context *c, int dir, int mod0, int limbs, 
         if (rc == KBFV_OK && t / reps < best_total) {
             best_total = t / reps;
             best_plan = encode_plan(log_n, 0, 0, x, 0);
+        }
+    }
+    // chunked two-pass transforms (both passes per chunk of at most 2^(k+23)
+    // bytes) with the picked tiles, timed as whole transforms against the
+    // unchunked one
+    if (rc == KBFV_OK && best_plan != 0 && !single_pass(log_n, decode_plan(log_n, best_plan))) {
+        const long N = c->n;
+        const long bytes = (long)polys * limbs * N * 8;
+        auto time_transform = [&](uint32_t plan, float *ms) -> int {
+            auto run = [&]() -> int {
+                return ntt_with_plan(c, dir, false, plan, scratch, scratch, mod0, limbs, polys, 1, 0,
+                                     (long)limbs * N, N, 0, (long)limbs * N, N, st);
+            };
+            KB_TRY(run());
+            KB_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; ++r) KB_TRY(run());
+            KB_HIP(hipEventRecord(e1, st));
+            KB_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            KB_HIP(hipEventElapsedTime(&t, e0, e1));
+            *ms = t / reps;
+            return KBFV_OK;
+        };
+        float whole = 0;
+        rc = time_transform(best_plan, &whole);
+        if (rc == KBFV_OK) best_total = whole;
+        for (int k = 2; k <= 6 && rc == KBFV_OK; ++k) {
+            if ((1L << (k + 23)) >= bytes) break;  // one chunk: the same launches as unchunked
+            float ms = 0;
+            const uint32_t p = (best_plan & 0x3fffu) | (uint32_t)k << 14;
+            rc = time_transform(p, &ms);
+            if (rc == KBFV_OK && ms < best_total) best_total = ms, best_plan = p;
         }
     }
     (void)hipEventDestroy(e0);

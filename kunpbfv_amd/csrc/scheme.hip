@@ -37,7 +37,9 @@ enum : uint64_t {
     TAG_RGSW_A = 9,
     TAG_RGSW_E = 10,
     TAG_GAL_A = 11,
-    TAG_GAL_E = 12
+    TAG_GAL_E = 12,
+    TAG_SWK_A = 13,
+    TAG_SWK_E = 14
 };
 
 // bump allocator over the caller's workspace (256-byte aligned slices); with a
@@ -402,30 +404,47 @@ extern "C" size_t kbfv_keygen_rlk_workspace_bytes(const kbfv_context *c) {
     return c ? rlk_layout(c, nullptr, nullptr, nullptr, nullptr, nullptr) : 0;
 }
 
+// key-switching key from src (NTT over Q) to s, sampled under (tag_a, tag_e):
+// digit d holds a_d uniform and b_d = e_d + P*src on the limbs of digit d - a_d*s,
+// converted to T block by block and NTT over T (include/KeySwitch_23.cuh:21-66)
+static int keygen_ksk(const kbfv_context *c, const uint8_t seed[32], uint64_t tag_a, uint64_t tag_e,
+                      const uint64_t *src_q, const uint64_t *sk, uint64_t *key, uint64_t *e, uint64_t *a,
+                      uint64_t *b, hipStream_t s) {
+    const long N = c->n;
+    const int pn = c->pn, qn = c->qn, pq = pn + qn, D = c->dnum, B = c->ri_blocks, tn = c->tn;
+    KB_TRY(kb_sample(c, 3, seed, tag_e, 0, c->oP, pq, e, 0, D, s));
+    KB_TRY(kb_ntt_inplace(c, +1, e, c->oP, pq, D, s));
+    KB_TRY(kb_sample(c, 0, seed, tag_a, 0, c->oP, pq, a, 0, D, s));
+    const long total = (long)D * pq * N;
+    hipLaunchKernelGGL(k_ksk_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e, a, sk, src_q, b, pn,
+                       qn, c->n, total);
+    KB_LAUNCH_CHECK();
+    KB_TRY(kb_ntt_inplace(c, -1, a, c->oP, pq, D, s));
+    KB_TRY(kb_ntt_inplace(c, -1, b, c->oP, pq, D, s));
+    const long half = (long)D * B * tn * N;
+    KB_TRY(kb_modup_pq_to_t(c, a, key, D, s));
+    KB_TRY(kb_modup_pq_to_t(c, b, key + half, D, s));
+    return kb_ntt_inplace(c, +1, key, c->oT, tn, 2 * D * B, s);
+}
+
 extern "C" int kbfv_keygen_rlk(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk, uint64_t *rlk,
                                void *ws, size_t ws_bytes, void *st) {
     KB_REQUIRE(c && seed && sk && rlk && ws, "null argument");
     KB_REQUIRE(ws_bytes >= kbfv_keygen_rlk_workspace_bytes(c), "workspace too small");
     uint64_t *s2, *e, *a, *b;
     rlk_layout(c, ws, &s2, &e, &a, &b);
-    const long N = c->n;
-    const int pn = c->pn, qn = c->qn, pq = pn + qn, D = c->dnum, B = c->ri_blocks, tn = c->tn;
-    const uint64_t *s_q = sk + (long)pn * N;
-    hipStream_t s = S(st);
-    KB_TRY(kb_poly_binop(c, 2, s_q, s_q, s2, c->oQ, qn, 1, s));
-    KB_TRY(kb_sample(c, 3, seed, TAG(TAG_RLK_E), 0, c->oP, pq, e, 0, D, s));
-    KB_TRY(kb_ntt_inplace(c, +1, e, c->oP, pq, D, s));
-    KB_TRY(kb_sample(c, 0, seed, TAG(TAG_RLK_A), 0, c->oP, pq, a, 0, D, s));
-    const long total = (long)D * pq * N;
-    hipLaunchKernelGGL(k_ksk_combine, dim3(blocks_for(total, BS)), dim3(BS), 0, s, c->t, e, a, sk, s2, b, pn, qn,
-                       c->n, total);
-    KB_LAUNCH_CHECK();
-    KB_TRY(kb_ntt_inplace(c, -1, a, c->oP, pq, D, s));
-    KB_TRY(kb_ntt_inplace(c, -1, b, c->oP, pq, D, s));
-    const long half = (long)D * B * tn * N;
-    KB_TRY(kb_modup_pq_to_t(c, a, rlk, D, s));
-    KB_TRY(kb_modup_pq_to_t(c, b, rlk + half, D, s));
-    return kb_ntt_inplace(c, +1, rlk, c->oT, tn, 2 * D * B, s);
+    const uint64_t *s_q = sk + (long)c->pn * c->n;
+    KB_TRY(kb_poly_binop(c, 2, s_q, s_q, s2, c->oQ, c->qn, 1, S(st)));
+    return keygen_ksk(c, seed, TAG(TAG_RLK_A), TAG(TAG_RLK_E), s2, sk, rlk, e, a, b, S(st));
+}
+
+extern "C" int kbfv_keygen_swk(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk,
+                               const uint64_t *src_q, uint64_t *key, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && seed && sk && src_q && key && ws, "null argument");
+    KB_REQUIRE(ws_bytes >= kbfv_keygen_rlk_workspace_bytes(c), "workspace too small");
+    uint64_t *s2, *e, *a, *b;
+    rlk_layout(c, ws, &s2, &e, &a, &b);
+    return keygen_ksk(c, seed, TAG(TAG_SWK_A), TAG(TAG_SWK_E), src_q, sk, key, e, a, b, S(st));
 }
 
 // ---------------------------------------------------------------- encrypt / decrypt
@@ -512,6 +531,34 @@ extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const
         MultWs w;
         mult_layout(c, n, ws, &w);
         KB_TRY(mult_chunk(c, rlk, ct1 + b0 * CTS, ct2 + b0 * CTS, out + b0 * CTS, n, w, S(st)));
+    }
+    return KBFV_OK;
+}
+
+// compute_t_QInv_c0c1c2 (include/KeySwitch_23.cuh:703-926), batched: steps 1-7
+// of mult_chunk, then the three scaled tensor polynomials back to the NTT domain
+extern "C" int kbfv_mult_tensor(const kbfv_context *c, const uint64_t *ct1, const uint64_t *ct2, uint64_t *out3,
+                                int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && ct1 && ct2 && out3 && ws && batch > 0, "null argument");
+    int cb = batch;
+    while (cb > 1 && mult_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
+    KB_REQUIRE(mult_layout(c, cb, nullptr, nullptr) <= ws_bytes, "workspace too small for one ciphertext");
+    const long N = c->n, qn = c->qn, rn = c->rn, sn = qn + rn, CTS = 2 * qn * N, D3S = 3 * qn * N;
+    hipStream_t s = S(st);
+    for (int b0 = 0; b0 < batch; b0 += cb) {
+        const int n = std::min(cb, batch - b0);
+        MultWs w;
+        mult_layout(c, n, ws, &w);
+        const uint64_t *a = ct1 + b0 * CTS, *b = ct2 + b0 * CTS;
+        KB_TRY(kb_intt_noscale(c, a, w.X1, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_intt_noscale(c, b, w.X2, c->oQ, qn, 2, n, CTS, qn * N, N, CTS, qn * N, N, s));
+        KB_TRY(kb_bconv(c, true, w.X1, qn * N, w.R1, rn * N, 4 * n, s, true));
+        KB_TRY(kb_ntt(c, +1, w.R1, w.R1, c->oR, rn, 4 * n, 1, 0, rn * N, N, 0, rn * N, N, s));
+        KB_TRY(kb_tensor_qr(c, a, b, w.R1, w.R2, w.D3, n, s));
+        KB_TRY(kb_ntt(c, -1, w.D3, w.D3, c->oQ, sn, 3 * n, 1, 0, sn * N, N, 0, sn * N, N, s));
+        KB_TRY(kb_scale_round(c, w.D3, w.SR, 3 * n, s));
+        KB_TRY(kb_bconv(c, false, w.SR, rn * N, w.DQ, qn * N, 3 * n, s));
+        KB_TRY(kb_ntt(c, +1, w.DQ, out3 + b0 * D3S, c->oQ, qn, 3 * n, 1, 0, qn * N, N, 0, qn * N, N, s));
     }
     return KBFV_OK;
 }

@@ -145,6 +145,11 @@ void or_keygen_rlk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_p
     free(s2);
 }
 
+void or_keygen_swk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq, const uint64_t *src_q,
+                   uint64_t *key_t) {
+    keygen_ksk(c, seed, OR_TAG(OR_TAG_SWK_A, 0), OR_TAG(OR_TAG_SWK_E, 0), src_q, sk_pq, key_t);
+}
+
 /* ---------------- Galois automorphisms and rotation ---------------- */
 uint64_t or_galois_elt(const or_ctx *c, long steps) {
     uint64_t m2 = 2 * (uint64_t)c->n;

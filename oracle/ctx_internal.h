@@ -8,6 +8,13 @@
 
 #define OR_MAXW 160 /* bigint words: 10240 bits */
 
+/* Planted mistakes for the tests that show the pins reject them
+ * (tests/test_oracle_mutations.py builds the sources with -DOR_MUTATION=n);
+ * 0 in every real build. */
+#ifndef OR_MUTATION
+#define OR_MUTATION 0
+#endif
+
 typedef struct {
     int n;
     uint64_t w[OR_MAXW];

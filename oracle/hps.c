@@ -17,10 +17,15 @@
  *
  * Pinned by (tests/test_oracle_hps.py): exact CRT in Python big integers.
  * bConv must equal the centered lift of x mod Q reduced mod r_j (or that lift
- * +-Q when the double estimate sits within 2^-40 of a rounding tie);
- * scale-and-round must equal round(t*x/Q) mod r_j (same tie caveat); decrypt
- * must equal round(t*x/Q) mod t; the tensor equals the mod-q products;
- * Delta-m equals ceil(Q*m/t) mod q_i.
+ * +-Q when the double estimate sits within 2^-30 of a rounding tie);
+ * scale-and-round must equal floor(t*X/Q + 1/2) mod r_j exactly when the Q
+ * residues are below 2^20 (the double estimate is then exact to 2^-27) and
+ * t*X/Q is further than 2^-20 from a half-integer, and the same up to one small
+ * offset common to every r_j on full-size residues (the estimate's 60-bit
+ * rounding); decrypt must equal round(t*x/Q) mod t (exactly on small residues
+ * away from ties); the tensor equals the mod-q products; Delta-m equals
+ * ceil(Q*m/t) mod q_i.  tests/test_oracle_mutations.py shows the checks reject
+ * a dropped rounding half and a rounded (not truncated) nu.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -92,11 +97,19 @@ void or_scale_round(const or_ctx *c, const uint64_t *x_qr, uint64_t *y_r) {
     size_t N = c->n;
     int qn = c->qn, rn = c->rn;
     for (size_t k = 0; k < N; k++) {
+#if OR_MUTATION == 1 /* planted mistake (tests only): the rounding half dropped */
+        double nu = 0.0;
+#else
         double nu = 0.5;
+#endif
         for (int i = 0; i < qn; i++) nu = fma((double)x_qr[(size_t)i * N + k], c->sr_frac[i], nu);
         /* the reference's static_cast<uint64> of nu is a saturating conversion
          * (nvcc's cvt.rzi.u64.f64): nu can exceed 2^64 when every x_i is near q_i */
+#if OR_MUTATION == 4 /* planted mistake (tests only): nu rounded instead of truncated */
+        uint64_t alpha = (uint64_t)llround(nu);
+#else
         uint64_t alpha = nu >= 18446744073709551616.0 ? UINT64_MAX : (uint64_t)nu;
+#endif
         for (int j = 0; j < rn; j++) {
             uint64_t rj = c->mod[c->oR + j];
             const uint64_t *row = c->sr_modr + (size_t)j * (qn + 1);

@@ -14,14 +14,20 @@
  * and the 128-bit wrap-around of `acc - vv_floor * (M mod m)` restated
  * exactly.
  *
- * Pinned by (tests/test_oracle_ks.py), Python big integers: modUpQjtoT at K=1
- * is x mod t_k exactly; the inner product is the mod-t_k dot product; T->PQ is
- * the centered CRT value (in (-T/2, T/2)) reduced mod each PQ prime -- the
- * float estimate cannot err there because the centred input keeps the
- * fractional part near 1/2; PQ->T equals the CRT lift of the block residues
- * (or that lift +- R_b, both congruent mod R_b); modDown equals
- * (x_q - [x]_P) * P^-1 mod q.  End to end, decrypt(relin(d2)) recovers
- * s^2 * d2 up to small noise (tests/test_oracle_bfv.py).
+ * Pinned by (tests/test_oracle_ks.py), Python big integers: Q_j->T and PQ->T
+ * equal the CRT value of the block residues in [0, Q_j) / [0, R_b) reduced mod
+ * each t_k, exactly, whenever the exact rational sum of [x_j * hat_j^-1]/m_j
+ * lies further than 2^-18 from an integer (the float estimate's error is below
+ * 2^-21 there; at K=1 the sum is below 1 and the result is x mod t_k always);
+ * the inner product is the mod-t_k dot product; T->PQ is the centered CRT
+ * value (in (-T/2, T/2)) reduced mod each PQ prime -- the float estimate
+ * cannot err there because the centred input keeps the fractional part near
+ * 1/2; modDown equals (x_q - S) * P^-1 mod q exactly, with S = x_p at K=1 and
+ * S = sum_i [x_i * phat_i^-1]_{p_i} * phat_i (not reduced mod P, as the
+ * reference's multi-prime kernel leaves it) at K>1.  tests/test_oracle_mutations.py
+ * shows these checks reject a rounded estimate and an added alpha*P
+ * correction.  End to end, decrypt(relin(d2)) recovers s^2 * d2 up to small
+ * noise (tests/test_oracle_bfv.py).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -29,6 +35,15 @@
 #include "ctx_internal.h"
 
 static uint64_t reduce128(u128 a, uint64_t m) { return (uint64_t)(a % m); }
+
+/* The Q_j -> T and PQ -> T conversions truncate the float overflow estimate
+ * (`(int)vv`, include/BasisConv.cuh:32,112). */
+#if OR_MUTATION == 2 /* planted mistake (tests only): the estimate rounded */
+#include <math.h>
+#define TRUNC_VV(v) ((int)lrintf(v))
+#else
+#define TRUNC_VV(v) ((int)(v))
+#endif
 
 /* x_q: [qn][N] coefficient form -> out_t: [dnum][tn][N] */
 void or_modup_qj_to_t(const or_ctx *c, const uint64_t *x_q, uint64_t *out_t) {
@@ -46,7 +61,7 @@ void or_modup_qj_to_t(const or_ctx *c, const uint64_t *x_q, uint64_t *out_t) {
                 float rr = (float)rb[j];
                 vv += rr / (float)qi;
             }
-            int vf = (int)vv;
+            int vf = TRUNC_VV(vv);
             for (int i = 0; i < tn; i++) {
                 uint64_t t = c->mod[c->oT + i];
                 u128 acc = 0;
@@ -72,7 +87,7 @@ void or_modup_pq_to_t(const or_ctx *c, const uint64_t *x_pq, uint64_t *out_t) {
                 rb[j] = or_mulmod(x_pq[(size_t)(b * g + j) * N + k], c->ri_hatinv[b * g + j], ri);
                 vv += (float)rb[j] / (float)ri;
             }
-            int vf = (int)vv;
+            int vf = TRUNC_VV(vv);
             for (int i = 0; i < tn; i++) {
                 uint64_t t = c->mod[c->oT + i];
                 u128 acc = 0;
@@ -156,6 +171,16 @@ void or_moddown(const or_ctx *c, const uint64_t *in_pq, uint64_t *out_q) {
                         acc += (u128)rb * c->p_hat_mod_q[l * pn + i];
                     }
                     ra = reduce128(acc, q);
+#if OR_MUTATION == 3 /* planted mistake (tests only): an alpha*P correction the reference does not make */
+                    double a = 0;
+                    for (int i = 0; i < pn; i++) {
+                        uint64_t p = c->mod[c->oP + i];
+                        a += (double)or_mulmod(in_pq[((size_t)h * pq + i) * N + k], c->p_hatinv[i], p) / (double)p;
+                    }
+                    uint64_t pm = 1;
+                    for (int i = 0; i < pn; i++) pm = or_mulmod(pm, c->mod[c->oP + i] % q, q);
+                    ra = (ra + q - or_mulmod((uint64_t)llround(a) % q, pm, q)) % q;
+#endif
                 }
                 ra = q - ra + in_pq[((size_t)h * pq + pn + l) * N + k];
                 out_q[((size_t)h * qn + l) * N + k] = or_mulmod(ra, c->Pinv_mod_q[l], q);

@@ -116,7 +116,9 @@ enum {
     OR_TAG_RGSW_A = 9,
     OR_TAG_RGSW_E = 10,
     OR_TAG_GAL_A = 11,
-    OR_TAG_GAL_E = 12
+    OR_TAG_GAL_E = 12,
+    OR_TAG_SWK_A = 13,
+    OR_TAG_SWK_E = 14
 };
 
 /* ---------------- BFV scheme (bfv.c) ---------------- */
@@ -124,6 +126,11 @@ void or_keygen_sk(const or_ctx *c, const uint8_t seed[32], uint64_t *sk_pq);
 void or_keygen_pk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq, uint64_t *pk);
 void or_keygen_rlk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq,
                    uint64_t *rlk_t);
+/* key-switching key from src_q ([qn][N], NTT over Q) to the secret, layout of
+ * the relinearisation key (the reference's addSWKey_23 is an unfilled stub,
+ * include/KeySwitch_23.cuh:6-16; this is addMultKey_23 with s^2 -> src) */
+void or_keygen_swk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_pq, const uint64_t *src_q,
+                   uint64_t *key_t);
 void or_encrypt(const or_ctx *c, const uint8_t seed[32], uint64_t entry, const uint64_t *pk,
                 const uint64_t *m, uint64_t *ct);
 void or_decrypt(const or_ctx *c, const uint64_t *sk_pq, const uint64_t *ct, uint64_t *m);

@@ -17,10 +17,10 @@ f64p = C.POINTER(C.c_double)
 u8p = C.POINTER(C.c_uint8)
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
-    lib = C.CDLL(LIB_PATH)
+def _load(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle library missing: {path} (run `make -C oracle`)")
+    lib = C.CDLL(path)
     sig = {
         "or_mulmod": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
         "or_powmod": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -68,6 +68,7 @@ def _load():
         "or_keygen_sk": (None, [C.c_void_p, u8p, u64p]),
         "or_keygen_pk": (None, [C.c_void_p, u8p, u64p, u64p]),
         "or_keygen_rlk": (None, [C.c_void_p, u8p, u64p, u64p]),
+        "or_keygen_swk": (None, [C.c_void_p, u8p, u64p, u64p, u64p]),
         "or_encrypt": (None, [C.c_void_p, u8p, C.c_uint64, u64p, u64p, u64p]),
         "or_decrypt": (None, [C.c_void_p, u64p, u64p, u64p]),
         "or_mult_relin": (None, [C.c_void_p, u64p, u64p, u64p, u64p]),
@@ -99,6 +100,12 @@ def _load():
 lib = _load()
 
 
+def load_variant(path):
+    """Bindings for another build of the oracle sources (tests that plant a
+    mistake in the oracle compile one with -DOR_MUTATION=<n>)."""
+    return _load(path)
+
+
 def P(a):
     """numpy uint64 array -> pointer (array must stay alive)."""
     assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
@@ -117,53 +124,55 @@ def u64(x):
 class Ctx:
     """Oracle context.  Moduli order [P | Q | R | T | t]."""
 
-    def __init__(self, log_n, q, p, t, gamma, plain):
+    def __init__(self, log_n, q, p, t, gamma, plain, _lib=None):
+        """_lib: another build of the oracle (load_variant); the default build otherwise."""
+        self._lib = _lib if _lib is not None else lib
         self.q_in, self.p_in, self.t_in = u64(q), u64(p), u64(t)
-        self.h = lib.or_ctx_create(log_n, P(self.q_in), len(q), P(self.p_in), len(p),
+        self.h = self._lib.or_ctx_create(log_n, P(self.q_in), len(q), P(self.p_in), len(p),
                                    P(self.t_in), len(t), gamma, plain)
         if not self.h:
             raise ValueError("oracle rejected the parameter set")
         info = (C.c_int * 16)()
-        lib.or_ctx_info(self.h, info)
+        self._lib.or_ctx_info(self.h, info)
         (self.log_n, self.n, self.qn, self.pn, self.tn, self.rn, self.gamma, self.dnum,
          self.ri_blocks, self.batching, self.nmod, self.oP, self.oQ, self.oR, self.oT,
          self.oPlain) = list(info)
         self.mod = np.zeros(self.nmod, dtype=np.uint64)
-        lib.or_ctx_moduli(self.h, P(self.mod))
+        self._lib.or_ctx_moduli(self.h, P(self.mod))
         self.psi = np.zeros(self.nmod, dtype=np.uint64)
-        lib.or_ctx_psi(self.h, P(self.psi))
+        self._lib.or_ctx_psi(self.h, P(self.psi))
         self.plain = int(plain)
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib.or_ctx_destroy(self.h)
+            self._lib.or_ctx_destroy(self.h)
             self.h = None
 
     def table(self, m, which):
-        ptr = lib.or_ctx_table(self.h, m, which)
+        ptr = self._lib.or_ctx_table(self.h, m, which)
         return np.ctypeslib.as_array(ptr, shape=(self.n,)).copy()
 
     def export(self, name):
         out = np.zeros(1 << 16, dtype=np.uint64)
-        n = lib.or_ctx_export(self.h, name.encode(), P(out), len(out))
+        n = self._lib.or_ctx_export(self.h, name.encode(), P(out), len(out))
         assert n > 0, name
         return out[:n].copy()
 
     def export_f64(self, name):
         out = np.zeros(4096, dtype=np.float64)
-        n = lib.or_ctx_export_f64(self.h, name.encode(), out.ctypes.data_as(f64p), len(out))
+        n = self._lib.or_ctx_export_f64(self.h, name.encode(), out.ctypes.data_as(f64p), len(out))
         assert n > 0, name
         return out[:n].copy()
 
     # ---- transforms ----
     def ntt_fwd(self, a, mod0, limbs, polys=1):
         a = u64(a).copy()
-        lib.or_ntt_fwd_range(self.h, P(a), mod0, limbs, polys)
+        self._lib.or_ntt_fwd_range(self.h, P(a), mod0, limbs, polys)
         return a
 
     def ntt_inv(self, a, mod0, limbs, polys=1):
         a = u64(a).copy()
-        lib.or_ntt_inv_range(self.h, P(a), mod0, limbs, polys)
+        self._lib.or_ntt_inv_range(self.h, P(a), mod0, limbs, polys)
         return a
 
     def _call(self, fn, inputs, out_shape):
@@ -173,40 +182,40 @@ class Ctx:
         return out
 
     def bconv_q_to_r(self, x):
-        return self._call(lib.or_bconv_q_to_r, [x], (self.rn, self.n))
+        return self._call(self._lib.or_bconv_q_to_r, [x], (self.rn, self.n))
 
     def bconv_r_to_q(self, x):
-        return self._call(lib.or_bconv_r_to_q, [x], (self.qn, self.n))
+        return self._call(self._lib.or_bconv_r_to_q, [x], (self.qn, self.n))
 
     def tensor(self, c1, c2, limbs, mod0):
         a, b = u64(c1), u64(c2)
         out = np.zeros((3, limbs, self.n), dtype=np.uint64)
-        lib.or_tensor(self.h, P(a), P(b), P(out), limbs, mod0)
+        self._lib.or_tensor(self.h, P(a), P(b), P(out), limbs, mod0)
         return out
 
     def scale_round(self, x):
-        return self._call(lib.or_scale_round, [x], (self.rn, self.n))
+        return self._call(self._lib.or_scale_round, [x], (self.rn, self.n))
 
     def hps_decrypt(self, x):
-        return self._call(lib.or_hps_decrypt, [x], (self.n,))
+        return self._call(self._lib.or_hps_decrypt, [x], (self.n,))
 
     def delta_m(self, m):
-        return self._call(lib.or_add_delta_m, [m], (self.qn, self.n))
+        return self._call(self._lib.or_add_delta_m, [m], (self.qn, self.n))
 
     def modup_qj_to_t(self, x):
-        return self._call(lib.or_modup_qj_to_t, [x], (self.dnum, self.tn, self.n))
+        return self._call(self._lib.or_modup_qj_to_t, [x], (self.dnum, self.tn, self.n))
 
     def modup_pq_to_t(self, x):
-        return self._call(lib.or_modup_pq_to_t, [x], (self.ri_blocks, self.tn, self.n))
+        return self._call(self._lib.or_modup_pq_to_t, [x], (self.ri_blocks, self.tn, self.n))
 
     def inner_product(self, cin, key):
-        return self._call(lib.or_inner_product, [cin, key], (2, self.ri_blocks, self.tn, self.n))
+        return self._call(self._lib.or_inner_product, [cin, key], (2, self.ri_blocks, self.tn, self.n))
 
     def modup_t_to_pq(self, x):
-        return self._call(lib.or_modup_t_to_pq, [x], (2, self.pn + self.qn, self.n))
+        return self._call(self._lib.or_modup_t_to_pq, [x], (2, self.pn + self.qn, self.n))
 
     def moddown(self, x):
-        return self._call(lib.or_moddown, [x], (2, self.qn, self.n))
+        return self._call(self._lib.or_moddown, [x], (2, self.qn, self.n))
 
     # ---- samplers ----
     def sample(self, kind, seed, tag, mod0, limbs, entry=0):
@@ -219,110 +228,117 @@ class Ctx:
     def keygen_sk(self, seed):
         s, sp = B(seed)
         sk = np.zeros((self.pn + self.qn, self.n), dtype=np.uint64)
-        lib.or_keygen_sk(self.h, sp, P(sk))
+        self._lib.or_keygen_sk(self.h, sp, P(sk))
         return sk
 
     def keygen_pk(self, seed, sk):
         s, sp = B(seed)
         sk = u64(sk)
         pk = np.zeros((2, self.qn, self.n), dtype=np.uint64)
-        lib.or_keygen_pk(self.h, sp, P(sk), P(pk))
+        self._lib.or_keygen_pk(self.h, sp, P(sk), P(pk))
         return pk
 
     def keygen_rlk(self, seed, sk):
         s, sp = B(seed)
         sk = u64(sk)
         rlk = np.zeros((2, self.dnum, self.ri_blocks, self.tn, self.n), dtype=np.uint64)
-        lib.or_keygen_rlk(self.h, sp, P(sk), P(rlk))
+        self._lib.or_keygen_rlk(self.h, sp, P(sk), P(rlk))
         return rlk
+
+    def keygen_swk(self, seed, sk, src_q):
+        s, sp = B(seed)
+        sk, src = u64(sk), u64(src_q)
+        key = np.zeros((2, self.dnum, self.ri_blocks, self.tn, self.n), dtype=np.uint64)
+        self._lib.or_keygen_swk(self.h, sp, P(sk), P(src), P(key))
+        return key
 
     def encrypt(self, seed, pk, m, entry=0):
         s, sp = B(seed)
         pk, m = u64(pk), u64(m)
         ct = np.zeros((2, self.qn, self.n), dtype=np.uint64)
-        lib.or_encrypt(self.h, sp, entry, P(pk), P(m), P(ct))
+        self._lib.or_encrypt(self.h, sp, entry, P(pk), P(m), P(ct))
         return ct
 
     def decrypt(self, sk, ct):
-        return self._call(lib.or_decrypt, [sk, ct], (self.n,))
+        return self._call(self._lib.or_decrypt, [sk, ct], (self.n,))
 
     def mult_relin(self, rlk, c1, c2):
-        return self._call(lib.or_mult_relin, [rlk, c1, c2], (2, self.qn, self.n))
+        return self._call(self._lib.or_mult_relin, [rlk, c1, c2], (2, self.qn, self.n))
 
     def mult_tensor(self, c1, c2):
-        return self._call(lib.or_mult_tensor, [c1, c2], (3, self.qn, self.n))
+        return self._call(self._lib.or_mult_tensor, [c1, c2], (3, self.qn, self.n))
 
     def relin(self, rlk, d2):
-        return self._call(lib.or_relin, [rlk, d2], (2, self.qn, self.n))
+        return self._call(self._lib.or_relin, [rlk, d2], (2, self.qn, self.n))
 
     def encode(self, slots):
-        return self._call(lib.or_encode, [slots], (self.n,))
+        return self._call(self._lib.or_encode, [slots], (self.n,))
 
     def decode(self, m):
-        return self._call(lib.or_decode, [m], (self.n,))
+        return self._call(self._lib.or_decode, [m], (self.n,))
 
     def rgsw_rows(self, base_bits):
-        return lib.or_rgsw_rows(self.h, base_bits)
+        return self._lib.or_rgsw_rows(self.h, base_bits)
 
     def rgsw_encrypt(self, seed, sk, m_coeff, base_bits):
         s, sp = B(seed)
         sk, m = u64(sk), u64(m_coeff)
         rows = self.rgsw_rows(base_bits)
         out = np.zeros((rows, 2, self.qn, self.n), dtype=np.uint64)
-        lib.or_rgsw_encrypt(self.h, sp, P(sk), P(m), base_bits, P(out))
+        self._lib.or_rgsw_encrypt(self.h, sp, P(sk), P(m), base_bits, P(out))
         return out
 
     def external_product(self, rgsw, ct, base_bits):
         rg, c = u64(rgsw), u64(ct)
         out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
-        lib.or_external_product(self.h, P(rg), P(c), base_bits, P(out))
+        self._lib.or_external_product(self.h, P(rg), P(c), base_bits, P(out))
         return out
 
     def negate(self, ct):
-        return self._call(lib.or_negate, [ct], (2, self.qn, self.n))
+        return self._call(self._lib.or_negate, [ct], (2, self.qn, self.n))
 
     def mul_scalar(self, ct, k, add=None):
         c = u64(ct)
         out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
         a = u64(add) if add is not None else None
-        lib.or_mul_scalar(self.h, P(c), k, P(a) if a is not None else None, P(out))
+        self._lib.or_mul_scalar(self.h, P(c), k, P(a) if a is not None else None, P(out))
         return out
 
     def add_plain(self, ct, m):
-        return self._call(lib.or_add_plain, [ct, m], (2, self.qn, self.n))
+        return self._call(self._lib.or_add_plain, [ct, m], (2, self.qn, self.n))
 
     def seal_index_map(self):
         out = np.zeros(self.n, dtype=np.uint64)
-        lib.or_seal_index_map(self.h, P(out))
+        self._lib.or_seal_index_map(self.h, P(out))
         return out
 
     def encode_seal(self, slots):
-        return self._call(lib.or_encode_seal, [slots], (self.n,))
+        return self._call(self._lib.or_encode_seal, [slots], (self.n,))
 
     def decode_seal(self, m):
-        return self._call(lib.or_decode_seal, [m], (self.n,))
+        return self._call(self._lib.or_decode_seal, [m], (self.n,))
 
     # ---- Galois automorphisms / rotation ----
     def galois_elt(self, steps):
-        return int(lib.or_galois_elt(self.h, steps))
+        return int(self._lib.or_galois_elt(self.h, steps))
 
     def automorphism(self, g, a, mod0, limbs, polys=1):
         a = u64(a)
         out = np.zeros_like(a)
-        lib.or_automorphism(self.h, g, P(a), P(out), mod0, limbs, polys)
+        self._lib.or_automorphism(self.h, g, P(a), P(out), mod0, limbs, polys)
         return out
 
     def keygen_galois(self, seed, sk, g):
         s, sp = B(seed)
         sk = u64(sk)
         key = np.zeros((2, self.dnum, self.ri_blocks, self.tn, self.n), dtype=np.uint64)
-        lib.or_keygen_galois(self.h, sp, P(sk), g, P(key))
+        self._lib.or_keygen_galois(self.h, sp, P(sk), g, P(key))
         return key
 
     def rotate(self, key, g, ct):
         k, c = u64(key), u64(ct)
         out = np.zeros((2, self.qn, self.n), dtype=np.uint64)
-        lib.or_rotate(self.h, P(k), g, P(c), P(out))
+        self._lib.or_rotate(self.h, P(k), g, P(c), P(out))
         return out
 
 

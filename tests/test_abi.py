@@ -68,3 +68,27 @@ def test_one_hip_runtime_whatever_the_import_order():
     count, path = out.stdout.split()
     assert count == "1", out.stdout
     assert "torch" in path, path
+
+
+def test_random_seeds_come_from_os_entropy():
+    """Default keys are drawn from fresh OS entropy (the reference's
+    std::random_device), never from a built-in constant: two seeds differ,
+    and so do seeds from two processes."""
+    import subprocess
+    import sys
+    import kunpbfv_amd as kb
+    a, b = kb.random_seed(), kb.random_seed()
+    assert len(a) == 32 and a != b and a != bytes(32)
+    code = "import kunpbfv_amd as kb; print(kb.random_seed().hex())"
+    outs = {subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                           check=True).stdout.strip() for _ in range(2)}
+    assert len(outs) == 2
+
+
+def test_drop_in_default_seeds_are_not_constants():
+    """The C++ drop-in's default SecretKey and BFVScheme seeds come from
+    kbfv_random_seed (no fixed seed constant is left in the source)."""
+    src = open(os.path.join(ROOT, "kunpbfv_amd", "csrc", "bfv_api.cpp")).read()
+    hdr = open(os.path.join(ROOT, "include", "kunpbfv", "bfv.hpp")).read()
+    assert src.count("kbfv_random_seed(") >= 2
+    assert "0x5345435245544b59" not in src and "0x6b756e7042465600" not in hdr

@@ -38,3 +38,19 @@ def test_drop_in_ops_driver(n):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "total_wrong=0" in r.stdout
 
+
+
+MEMBERS = os.path.join(ROOT, "kunpbfv_amd", "examples", "batch_members")
+
+
+@pytest.mark.parametrize("n", [4096, 16384])
+def test_batch_and_scheme_members_driver(n):
+    """multAndEqual_23_batch equals mult_23 on every pair (two calls, the
+    second smaller, no device allocation after the first), encryptZero,
+    compute_t_QInv_c0c1c2 with hps_decrypt_scale_and_round, addSWKey_23,
+    addAutoKey_23 / automorphismAndAdd and the vector rotKey_vec_23"""
+    assert os.path.exists(MEMBERS), "example driver not built (run __graft_entry__.build())"
+    r = subprocess.run([MEMBERS, str(n)], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "total_wrong=0" in r.stdout

@@ -130,3 +130,31 @@ def test_autotune_json_cache(tmp_path):
     assert json.load(open(path))[key][0][2] != 1 << 20
     ctx2.autotune(8, reps=1, cache=path)
     assert len(json.load(open(path))) == 2
+
+
+@pytest.mark.parametrize("layout", ["polys", "batch"])
+def test_chunked_plans_match_oracle(layout):
+    """Chunked two-pass plans (both passes per chunk of 32 MB) over a 36 MB
+    launch at N = 2^16, chunked along the batch or the polynomials: the
+    oracle's forward NTT and the exact inverse, with a short last chunk."""
+    kb, ctx, oc = make(16)
+    rng = np.random.default_rng(300)
+    mod0, limbs, n = ctx.oQ, ctx.qn, 24
+    x = rand_limbs(rng, ctx.mod[mod0:mod0 + limbs], ctx.n, n)  # [24][3][N] = 36 MiB
+    want = oc.ntt_fwd(x, mod0, limbs, n)
+    N = ctx.n
+    for p in (0x2cc8, 0x3cd8 | 1 << 12):
+        plan = (p & 0x3fff) | 2 << 14
+        assert kb.lib.kbfv_ntt_plan_valid(16, plan) == 0
+        set_all(kb, ctx, plan)
+        xd = kb.to_device(x)
+        if layout == "polys":
+            ctx.ntt(xd, xd, +1, mod0, limbs, polys=n)
+            assert np.array_equal(kb.to_host(xd), want), hex(plan)
+            ctx.ntt(xd, xd, -1, mod0, limbs, polys=n)
+        else:
+            s = (limbs * N, 0, N)
+            ctx.ntt(xd, xd, +1, mod0, limbs, polys=1, batch=n, s=s)
+            assert np.array_equal(kb.to_host(xd), want), hex(plan)
+            ctx.ntt(xd, xd, -1, mod0, limbs, polys=1, batch=n, s=s)
+        assert np.array_equal(kb.to_host(xd), x), hex(plan)

@@ -248,3 +248,40 @@ def test_external_product_matches_oracle(pair, base):
         want[3:] = mb[:-3]
         want[:3] = -mb[-3:]
         assert np.array_equal(dec[b].astype(object) % ctx.plain, want % ctx.plain), b
+
+
+def test_mult_tensor_and_switching_key_match_oracle(pair):
+    """compute_t_QInv_c0c1c2 (kbfv_mult_tensor: the scaled tensor in NTT form)
+    and addSWKey_23 (kbfv_keygen_swk) against the oracle; a switching key from
+    s^2 relinearises like the relinearisation key."""
+    import kunpbfv_amd as kb
+    ctx, oc = pair
+    s = [bytes([i]) * 32 for i in (51, 52, 53, 54)]
+    sk = ctx.keygen_sk(s[0])
+    pk = ctx.keygen_pk(s[1], sk)
+    rng = np.random.default_rng(55)
+    m = rng.integers(0, ctx.plain, size=(3, ctx.n), dtype=np.uint64)
+    md = kb.to_device(m)
+    ctx.encode(md)
+    ct = ctx.encrypt(s[2], pk, md)
+    cth = kb.to_host(ct)
+    c2 = ct[[1, 2, 0]].contiguous()
+    d3 = kb.to_host(ctx.mult_tensor(ct, c2))
+    for b, b2 in ((0, 1), (1, 2), (2, 0)):
+        want = oc.ntt_fwd(oc.mult_tensor(cth[b], cth[b2]), oc.oQ, oc.qn, 3)
+        assert np.array_equal(d3[b], want), b
+    # switching key from s^2 (NTT over Q), bit-exact, and usable as a relinearisation key
+    s_q = sk[ctx.pn:].contiguous()
+    s2 = ctx.empty(ctx.qn, ctx.n)
+    ctx.poly_op("mul", s_q, s_q, s2, ctx.oQ, ctx.qn)
+    swk = ctx.keygen_swk(s[3], sk, s2)
+    o_sk = oc.keygen_sk(s[0])
+    o_s2 = kb.to_host(s2)
+    assert np.array_equal(kb.to_host(swk), oc.keygen_swk(s[3], o_sk, o_s2))
+    prod = ctx.mult_relin(swk, ct[0:1].contiguous(), c2[0:1].contiguous())
+    dec = ctx.decrypt(sk, prod)
+    ctx.decode(dec)
+    got = kb.to_host(dec)[0]
+    assert ctx.batching
+    assert np.array_equal(got.astype(object), (m[0].astype(object) * m[1].astype(object)) % ctx.plain)
+    assert np.array_equal(kb.to_host(prod)[0], oc.mult_relin(kb.to_host(swk), cth[0], cth[1]))

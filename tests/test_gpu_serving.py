@@ -76,3 +76,15 @@ def test_external_product_by_x_shifts_the_message(rings):
     want[0] = -enc[-1]
     assert np.array_equal(got % ring.ctx.plain, want % ring.ctx.plain)
     assert m.shape == got.shape
+
+
+def test_large_rings_mix_matches_alone():
+    """The rings N = 2^14, 2^15 and 2^16 with their 128-bit chains (6, 13 and
+    28 + 1 primes): a seeded mix, every result equal to the same request alone."""
+    from kunpbfv_amd import serving
+    big = {ln: serving.Ring(ln, pool=3, max_batch=8) for ln in (14, 15, 16)}
+    reqs = serving.poisson_requests(qps=2000, duration_s=0.02, log_ns=[14, 15, 16], seed=5, pool=3)
+    assert len(reqs) >= 20 and {r.log_n for r in reqs} == {14, 15, 16}
+    stats = serving.serve(big, reqs, keep=True)
+    assert stats["requests"] == len(reqs)
+    _check_alone(big, reqs)

@@ -1,6 +1,8 @@
 """NTT plan space (host only, no device calls): the candidate list holds the
 built-in plan's shape and exactly the (split, tile, grid order) combinations
 whose passes the kernels cover, and the validator rejects malformed words."""
+import os
+
 import pytest
 
 
@@ -59,7 +61,9 @@ def test_large_rings_offer_radix32_splits():
 def test_validator_rejects_malformed_words():
     import kunpbfv_amd as kb
     lib = kb.lib
-    assert lib.kbfv_ntt_plan_valid(16, 1 << 14 | 8 | 12 << 4 | 12 << 8) != 0   # unknown bit
+    assert lib.kbfv_ntt_plan_valid(16, 1 << 17 | 8 | 12 << 4 | 12 << 8) != 0   # unknown bit
+    assert lib.kbfv_ntt_plan_valid(16, 1 << 14 | 8 | 12 << 4 | 12 << 8) != 0   # chunk code 1 (8 MB) unused
+    assert lib.kbfv_ntt_plan_valid(16, 7 << 14 | 8 | 12 << 4 | 12 << 8) != 0   # chunk code 7 unused
     assert lib.kbfv_ntt_plan_valid(16, 10 | 13 << 4 | 13 << 8) != 0           # split too far off
     assert lib.kbfv_ntt_plan_valid(12, 6 | 13 << 4 | 12 << 8) != 0            # tile larger than the ring
     assert lib.kbfv_ntt_plan_valid(11, 0) != 0                                 # ring degree
@@ -86,3 +90,35 @@ def test_plan_cache_file_round_trip(tmp_path):
     kb._cache_write(path, "c", [])
     assert kb._cache_read(path) == {"c": []}
     assert not [f for f in (tmp_path / "sub").iterdir() if ".tmp" in f.name]
+
+
+def test_chunked_plans_are_valid_two_pass_plans():
+    """Chunk codes 2..6 (both passes per chunk of 2^(k+23) bytes) are valid on
+    every two-pass candidate and on none of the single-pass plans."""
+    import kunpbfv_amd as kb
+    for log_n in (12, 14, 16):
+        for p in kb.ntt_plan_candidates(log_n):
+            single = (p & 15) == log_n
+            for k in range(2, 7):
+                ok = kb.lib.kbfv_ntt_plan_valid(log_n, p | k << 14) == 0
+                assert ok != single, (log_n, hex(p), k)
+                assert kb.ntt_plan_chunk_bytes(p | k << 14) == 1 << (k + 23)
+            assert kb.ntt_plan_chunk_bytes(p) == 0
+
+
+def test_plan_cache_concurrent_writers_keep_every_entry(tmp_path):
+    """Processes adding entries at once lose none of them (the cache is updated
+    under a file lock), and the cache key names the library build."""
+    import subprocess
+    import sys
+    import kunpbfv_amd as kb
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = str(tmp_path / "plans.json")
+    code = ("import sys, kunpbfv_amd as kb\n"
+            "for i in range(40): kb._cache_write(sys.argv[1], f'{sys.argv[2]}-{i}', [i])\n")
+    procs = [subprocess.Popen([sys.executable, "-c", code, path, f"w{j}"], cwd=root) for j in range(4)]
+    assert all(p.wait() == 0 for p in procs)
+    data = kb._cache_read(path)
+    assert set(data) == {f"w{j}-{i}" for j in range(4) for i in range(40)}
+    bid = kb.library_build_id()
+    assert len(bid) == 16 and bid == kb.library_build_id()

@@ -67,11 +67,50 @@ def test_bconv_r_to_q_is_centered_lift(oc):
         assert [int(v) for v in y[:, k]] == [Xc % q for q in Q]
 
 
+def check_scale_round_exact_small(oc, seed=13):
+    """Q residues below 2^20 and arbitrary R residues: the double estimate of
+    sum_i x_i * frac_i is then exact to about 2^-27 (each term errs by at most
+    x_i * 2^-54, each fma by half an ulp of a sum below 2^25), so away from a
+    rounding tie the result must be exactly floor(t X / Q + 1/2) mod r_j, X the
+    CRT value over Q u R.  A dropped +1/2, or nu rounded instead of truncated
+    (include/poly_arithmetic.cuh:1276-1290), moves about half of the outputs."""
+    from fractions import Fraction
+    rng = np.random.default_rng(seed)
+    S = [int(x) for x in oc.mod[oc.oQ:oc.oQ + oc.qn + oc.rn]]
+    R = S[oc.qn:]
+    Qp = 1
+    for q in S[:oc.qn]:
+        Qp *= q
+    x = np.stack([rng.integers(0, 1 << 20, size=oc.n, dtype=np.uint64) for _ in range(oc.qn)]
+                 + [rng.integers(0, r, size=oc.n, dtype=np.uint64) for r in R])
+    x[:oc.qn, :8] = 0  # X = the R part alone: t X / Q integral or far from a tie
+    y = oc.scale_round(x)
+    t = oc.plain
+    checked = 0
+    for k in range(0, oc.n, 3):
+        X, _ = crt(x[:, k], S)
+        f = Fraction(t * X, Qp) % 1
+        if abs(f - Fraction(1, 2)) < Fraction(1, 1 << 20):
+            continue
+        want = (2 * t * X + Qp) // (2 * Qp)
+        got = [int(v) for v in y[:, k]]
+        assert got == [want % r for r in R], (k, got, [want % r for r in R])
+        checked += 1
+    assert checked > 1000
+    return checked
+
+
+def test_scale_round_exact_on_small_residues(oc):
+    check_scale_round_exact_small(oc)
+
+
 def test_scale_round_is_round_t_x_over_q(oc):
-    """floor(t X / Q + 1/2) in every R prime, up to one small offset d common to
-    all R primes: the reference estimates the rounding term from 60-bit
-    residues in double precision (include/poly_arithmetic.cuh:1276-1282), so
-    the estimate is off by a few units, which HPS absorbs as noise."""
+    """Full-size residues: floor(t X / Q + 1/2) in every R prime, up to one
+    small offset d common to all R primes.  The reference estimates the
+    rounding term from 60-bit residues in double precision
+    (include/poly_arithmetic.cuh:1276-1282); a product x_i * frac_i near 2^60
+    carries rounding errors of a few hundred units, which HPS absorbs as noise
+    (the exact case is test_scale_round_exact_on_small_residues)."""
     rng = np.random.default_rng(3)
     S = [int(x) for x in oc.mod[oc.oQ:oc.oQ + oc.qn + oc.rn]]
     R = S[oc.qn:]
@@ -107,6 +146,26 @@ def test_hps_decrypt_recovers_messages(oc):
     X = [((Qp * int(mi) + t // 2) // t + ei * (Qp >> 130)) % Qp for mi, ei in zip(m, e)]
     x = np.array([[v % q for v in X] for q in Q], dtype=np.uint64)
     assert np.array_equal(oc.hps_decrypt(x), m.astype(np.uint64))
+
+
+def test_hps_decrypt_exact_on_small_residues(oc):
+    """Residues below 2^20: the double sums are exact to far below 2^-20, so
+    the result is round(t X / Q) mod t exactly away from a tie."""
+    from fractions import Fraction
+    rng = np.random.default_rng(14)
+    Q = [int(x) for x in oc.mod[oc.oQ:oc.oQ + oc.qn]]
+    x = np.stack([rng.integers(0, 1 << 20, size=oc.n, dtype=np.uint64) for _ in Q])
+    m = oc.hps_decrypt(x)
+    t = oc.plain
+    checked = 0
+    for k in range(0, oc.n, 3):
+        X, Qp = crt(x[:, k], Q)
+        f = Fraction(t * X, Qp) % 1
+        if abs(f - Fraction(1, 2)) < Fraction(1, 1 << 20):
+            continue
+        assert int(m[k]) % t == ((2 * t * X + Qp) // (2 * Qp)) % t, k
+        checked += 1
+    assert checked > 1000
 
 
 def test_hps_decrypt_uniform_inputs_round_within_estimate(oc):

@@ -32,34 +32,59 @@ def mods(oc, lo, n):
     return [int(x) for x in oc.mod[lo:lo + n]]
 
 
+def exact_block_lift(res, blk):
+    """CRT value X in [0, M) of the residues `res` of one block of moduli, and
+    the distance of the exact rational sum_j [x_j * (M/m_j)^-1]_{m_j} / m_j to
+    the nearest integer (the quantity the float estimate truncates)."""
+    from fractions import Fraction
+    M = 1
+    for m in blk:
+        M *= m
+    s = Fraction(0)
+    for r, m in zip(res, blk):
+        rb = int(r) * pow(M // m, -1, m) % m
+        s += Fraction(rb, m)
+    f = s - (s.numerator // s.denominator)
+    X, _ = crt(res, blk)
+    return X, min(f, 1 - f)
+
+
+def check_modup_to_t(oc, which, seed):
+    """Q_j -> T (which = "qj") or PQ -> T ("pq"): every output equals the block's
+    CRT value in [0, M_b) mod each t_k, exactly, unless the exact overflow sum
+    lies within 2^-18 of an integer (float estimate error < 2^-21 at <= 4 terms).
+    A rounded instead of truncated estimate gives X - M_b on about half of the
+    coefficients (include/BasisConv.cuh:32,112)."""
+    rng = np.random.default_rng(seed)
+    T = mods(oc, oc.oT, oc.tn)
+    if which == "qj":
+        src, width, blocks = mods(oc, oc.oQ, oc.qn), oc.pn, oc.dnum
+        x = np.stack([rng.integers(0, q, size=oc.n, dtype=np.uint64) for q in src])
+        out = oc.modup_qj_to_t(x)
+    else:
+        src, width, blocks = mods(oc, 0, oc.pn + oc.qn), oc.gamma, oc.ri_blocks
+        x = np.stack([rng.integers(0, m, size=oc.n, dtype=np.uint64) for m in src])
+        out = oc.modup_pq_to_t(x)
+    checked = skipped = 0
+    for b in range(blocks):
+        blk = src[b * width:(b + 1) * width]
+        for k in range(0, oc.n, 7):
+            X, dist = exact_block_lift(x[b * width:b * width + len(blk), k], blk)
+            if len(blk) > 1 and dist < 2.0 ** -18:
+                skipped += 1
+                continue
+            assert [int(v) for v in out[b, :, k]] == [X % t for t in T], (which, b, k)
+            checked += 1
+    assert checked > 200 and skipped * 100 < checked
+    return checked
+
+
 def test_modup_qj_to_t(oc):
-    rng = np.random.default_rng(1)
-    Q, T = mods(oc, oc.oQ, oc.qn), mods(oc, oc.oT, oc.tn)
-    x = np.stack([rng.integers(0, q, size=oc.n, dtype=np.uint64) for q in Q])
-    out = oc.modup_qj_to_t(x)
-    for d in range(oc.dnum):
-        blk = Q[d * oc.pn:(d + 1) * oc.pn]
-        for k in range(0, oc.n, 31):
-            X, Qj = crt(x[d * oc.pn:d * oc.pn + len(blk), k], blk)
-            got = [int(v) for v in out[d, :, k]]
-            if oc.pn == 1:
-                assert got == [X % t for t in T]  # exact at K = 1
-            else:  # float estimate: X or X + Qj, both congruent mod Qj
-                assert got in ([X % t for t in T], [(X + Qj) % t for t in T], [(X - Qj) % t for t in T])
+    check_modup_to_t(oc, "qj", 1)
 
 
 def test_modup_pq_to_t(oc):
-    rng = np.random.default_rng(2)
-    pq = oc.pn + oc.qn
-    PQ, T = mods(oc, 0, pq), mods(oc, oc.oT, oc.tn)
-    x = np.stack([rng.integers(0, m, size=oc.n, dtype=np.uint64) for m in PQ])
-    out = oc.modup_pq_to_t(x)
-    for b in range(oc.ri_blocks):
-        blk = PQ[b * oc.gamma:(b + 1) * oc.gamma]
-        for k in range(0, oc.n, 37):
-            X, Rb = crt(x[b * oc.gamma:b * oc.gamma + len(blk), k], blk)
-            got = [int(v) for v in out[b, :, k]]
-            assert got in ([X % t for t in T], [(X + Rb) % t for t in T], [(X - Rb) % t for t in T])
+    check_modup_to_t(oc, "pq", 2)
 
 
 def test_inner_product_is_mod_t_dot_product(oc):
@@ -100,8 +125,13 @@ def test_modup_t_to_pq_is_centered_crt(oc):
                     assert int(out[h, j, k]) == vals[h][b][k] % PQ[j]
 
 
-def test_moddown(oc):
-    rng = np.random.default_rng(5)
+def check_moddown(oc, seed=5):
+    """out = (x_q - S) * P^-1 mod q exactly, S = x_p mod q at K = 1 and, at
+    K > 1, S = sum_i [x_i * (P/p_i)^-1]_{p_i} * (P/p_i): the reference's
+    multi-prime kernel accumulates that sum in 128 bits and reduces it mod q
+    with no overflow correction (include/BasisConv.cuh:258-298), so S may
+    exceed [x]_P by up to (K-1) P and the result is still one exact value."""
+    rng = np.random.default_rng(seed)
     pq = oc.pn + oc.qn
     PQ = mods(oc, 0, pq)
     P, Q = PQ[:oc.pn], PQ[oc.pn:]
@@ -110,12 +140,17 @@ def test_moddown(oc):
         Pp *= p
     x = np.stack([np.stack([rng.integers(0, m, size=oc.n, dtype=np.uint64) for m in PQ]) for _ in range(2)])
     out = oc.moddown(x)
+    over = 0
     for h in range(2):
-        for k in range(0, oc.n, 19):
-            xp, _ = crt(x[h, :oc.pn, k], P)
+        for k in range(0, oc.n, 7):
+            S = sum((int(x[h, i, k]) * pow(Pp // p, -1, p) % p) * (Pp // p) for i, p in enumerate(P))
+            over += S >= Pp
             for l, q in enumerate(Q):
-                xq = int(x[h, oc.pn + l, k])
-                got = int(out[h, l, k])
-                # (x_q - [x]_P) / P, with [x]_P exact or off by a multiple v*P, 0 <= v < K
-                cands = [((xq - xp - v * Pp) * pow(Pp, -1, q)) % q for v in range(oc.pn)]
-                assert got in cands
+                want = ((int(x[h, oc.pn + l, k]) - S) * pow(Pp, -1, q)) % q
+                assert int(out[h, l, k]) == want, (h, l, k)
+    if oc.pn > 1:
+        assert over > 0  # the sum exceeded P somewhere: an added correction would show
+
+
+def test_moddown(oc):
+    check_moddown(oc)
