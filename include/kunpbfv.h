@@ -263,6 +263,24 @@ int kbfv_mult_relin(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t
                     const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
                     size_t ws_bytes, void *stream);
 
+/* Allocation-free HomMult + relinearisation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and bits as kbfv_mult_relin, but the intermediates live
+ * in a fixed device array of KBFV_NOALLOC_SCRATCH_BYTES per GPU (part of the
+ * library's code object, held until the process ends; its address is resolved
+ * per device at context creation), and the call allocates nothing.  The batch
+ * runs in chunks of kbfv_mult_noalloc_max_batch(ctx) ciphertexts; a shape
+ * whose single ciphertext needs more scratch than the array holds (N = 2^16
+ * with 28 Q primes needs about 302 MB) is rejected with KBFV_EINVAL.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams or host threads) race and must be serialised by the
+ * caller.  Nothing in the array needs to start at zero. */
+#define KBFV_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+size_t kbfv_mult_noalloc_max_batch(const kbfv_context *ctx);
+int kbfv_mult_relin_noalloc(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t *ct1,
+                            const uint64_t *ct2, uint64_t *out, int batch, void *stream);
+
 /* BFVScheme::compute_t_QInv_c0c1c2 (include/KeySwitch_23.cuh:703-926), batched:
  * the HPS-scaled tensor of each pair, out3 [batch][3][qn][N] = (a1a2 = axax_mul,
  * a1b2 + a2b1 = axbx1_mul, b1b2 = bxbx_mul), NTT form over Q; without the

@@ -75,6 +75,8 @@ SIGNATURES = {
     "kbfv_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
+    "kbfv_mult_noalloc_max_batch": (_sz, [_vp]),
+    "kbfv_mult_relin_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_negate": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mul_scalar": (C.c_int, [_vp, _vp, _u64, _vp, _vp, C.c_int, _vp]),
     "kbfv_add_plain_workspace_bytes": (_sz, [_vp, C.c_int]),

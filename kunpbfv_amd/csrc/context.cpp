@@ -636,6 +636,13 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     size_t o_iw1 = blob.add(inv_w1);
 
     hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) {
+        const int rc = kb_noalloc_setup(c->device);
+        if (rc != KBFV_OK) {
+            delete c;
+            return rc;
+        }
+    }
     if (e == hipSuccess) e = hipMalloc(&c->d_blob, blob.bytes.size());
     if (e == hipSuccess) e = hipMemcpy(c->d_blob, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void **)&c->d_tw, tw.size() * sizeof(kbfv_tw));

@@ -127,6 +127,10 @@ struct kbfv_context {
     std::atomic<uint32_t> ntt_plan[2][KBFV_NTT_BUCKETS] = {};
 };
 
+// resolves the allocation-free HomMult's scratch address on `device` (which
+// must be current); once per device, at context creation (scheme.hip)
+int kb_noalloc_setup(int device);
+
 // per-limb modulus helpers
 static inline uint64_t kb_host_mulmod(uint64_t a, uint64_t b, uint64_t q) {
     return (uint64_t)(((kb_u128_host)a * b) % q);

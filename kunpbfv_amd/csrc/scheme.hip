@@ -13,6 +13,7 @@
 // workspace (sizes from *_workspace_bytes), so a captured graph can replay it.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "kbfv_internal.h"
 #include "kbfv_ops.h"
@@ -514,10 +515,11 @@ extern "C" size_t kbfv_mult_workspace_bytes(const kbfv_context *c, int batch) {
     return (c && batch > 0) ? mult_layout(c, batch, nullptr, nullptr) : 0;
 }
 
-extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1,
-                               const uint64_t *ct2, uint64_t *out, int batch, void *ws, size_t ws_bytes,
-                               void *st) {
-    KB_REQUIRE(c && rlk && ct1 && ct2 && out && ws && batch > 0, "null argument");
+namespace {
+// batched HomMult + relinearisation over a workspace of ws_bytes, in chunks of
+// as many ciphertexts as it holds
+int mult_relin_ws(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1, const uint64_t *ct2,
+                  uint64_t *out, int batch, void *ws, size_t ws_bytes, hipStream_t st) {
     KB_REQUIRE(c->rn <= 64, "too many limbs");
     // largest chunk that fits the workspace
     int cb = batch;
@@ -530,9 +532,69 @@ extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const
         // input's polynomials (X2, R2) as following the first's directly
         MultWs w;
         mult_layout(c, n, ws, &w);
-        KB_TRY(mult_chunk(c, rlk, ct1 + b0 * CTS, ct2 + b0 * CTS, out + b0 * CTS, n, w, S(st)));
+        KB_TRY(mult_chunk(c, rlk, ct1 + b0 * CTS, ct2 + b0 * CTS, out + b0 * CTS, n, w, st));
     }
     return KBFV_OK;
+}
+}  // namespace
+
+// Scratch of the allocation-free HomMult (kbfv_mult_relin_noalloc): one copy
+// per GPU, loaded with this file's code object and held until the process
+// ends.  In host code `scratch` is only a host-side stand-in for the symbol;
+// its device address on each GPU comes from hipGetSymbolAddress, resolved
+// once per device by kb_noalloc_setup (called at context creation with that
+// device current) and kept in noalloc_ptr.
+static __device__ __align__(256) uint8_t scratch[KBFV_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+void *noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+int kb_noalloc_setup(int device) {
+    KB_REQUIRE(device >= 0 && device < kMaxDevices, "device index %d out of range", device);
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device]) return KBFV_OK;
+    int cur = -1;
+    KB_HIP(hipGetDevice(&cur));
+    KB_REQUIRE(cur == device, "scratch set-up needs device %d current (current: %d)", device, cur);
+    void *ptr = nullptr;
+    KB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    KB_REQUIRE(ptr != nullptr, "no device address for the scratch array");
+    noalloc_ptr[device] = ptr;
+    return KBFV_OK;
+}
+
+extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1,
+                               const uint64_t *ct2, uint64_t *out, int batch, void *ws, size_t ws_bytes,
+                               void *st) {
+    KB_REQUIRE(c && rlk && ct1 && ct2 && out && ws && batch > 0, "null argument");
+    return mult_relin_ws(c, rlk, ct1, ct2, out, batch, ws, ws_bytes, S(st));
+}
+
+extern "C" size_t kbfv_mult_noalloc_max_batch(const kbfv_context *c) {
+    if (!c) return 0;
+    size_t cb = 0;
+    while (cb < (1u << 20) && mult_layout(c, (int)cb + 1, nullptr, nullptr) <= KBFV_NOALLOC_SCRATCH_BYTES) ++cb;
+    return cb;
+}
+
+extern "C" int kbfv_mult_relin_noalloc(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1,
+                                       const uint64_t *ct2, uint64_t *out, int batch, void *st) {
+    KB_REQUIRE(c && rlk && ct1 && ct2 && out && batch > 0, "null argument");
+    const size_t one = mult_layout(c, 1, nullptr, nullptr);
+    KB_REQUIRE(one <= KBFV_NOALLOC_SCRATCH_BYTES,
+               "HomMult of one ciphertext at N=%d with %d Q primes needs %zu bytes of scratch; the "
+               "allocation-free entry point holds %zu (use kbfv_mult_relin with a workspace)",
+               c->n, c->qn, one, (size_t)KBFV_NOALLOC_SCRATCH_BYTES);
+    void *ws = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(noalloc_mu);
+        ws = (c->device >= 0 && c->device < kMaxDevices) ? noalloc_ptr[c->device] : nullptr;
+    }
+    KB_REQUIRE(ws != nullptr, "scratch of device %d was not set up at context creation", c->device);
+    return mult_relin_ws(c, rlk, ct1, ct2, out, batch, ws, KBFV_NOALLOC_SCRATCH_BYTES, S(st));
 }
 
 // compute_t_QInv_c0c1c2 (include/KeySwitch_23.cuh:703-926), batched: steps 1-7
