@@ -96,7 +96,7 @@ def cpu_baseline(ctx, ct1, ct2, rlk_shape):
     t0 = time.perf_counter()
     oc.mult_relin(rlk, a, b)
     dt1 = time.perf_counter() - t0
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = host_threads()
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
         list(ex.map(lambda _: oc.mult_relin(rlk, a, b), range(threads)))
@@ -104,7 +104,72 @@ def cpu_baseline(ctx, ct1, ct2, rlk_shape):
     return {"value": round(threads / dtn, 4), "unit": "HomMult+relin/s", "cores": threads, "kind": "port",
             "sample": f"{threads} independent HomMult+relin at N=2^{ctx.log_n}, {ctx.qn}+{ctx.pn} primes, "
                       f"CPU oracle, one per thread, {dtn:.2f} s; one core alone: {dt1:.2f} s per op",
-            "single_core_value": round(1.0 / dt1, 4), "cpu_model": cpu_model()}
+            "single_core_value": round(1.0 / dt1, 4), "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def host_threads():
+    """Threads for the CPU baseline: every CPU this process may run on, capped
+    at the 16 host cores a one-GPU box allots to its GPU's jobs."""
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def cpu_c1(threads, seconds=1.0):
+    """The CPU-runnable BFV workload at N=4096 (two 36-bit Q primes and one
+    37-bit special prime, 1 mod 8192, t=65537, batching): encrypt, add,
+    mult+relin and decrypt of seeded uniform slot vectors, each timed on the
+    CPU oracle for about `seconds` on one core and then with one independent
+    stream of operations per host thread.  The reference's own timing loops
+    (perf_encrypt.cu:26-40, perf_add.cu:31-51, perf_mult.cu:29-45) time the
+    same four operations one call at a time."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    import params as tp
+    d = tp.c1_params()
+    oc = orc.Ctx(d["log_n"], d["q"], d["p"], d["t"], d["gamma"], d["plain"])
+    rng = np.random.default_rng(4096)
+    sk = oc.keygen_sk(bytes([71]) * 32)
+    pk = oc.keygen_pk(bytes([72]) * 32, sk)
+    rlk = oc.keygen_rlk(bytes([73]) * 32, sk)
+    m1, m2 = (oc.encode(rng.integers(0, d["plain"], size=oc.n, dtype=np.uint64)) for _ in range(2))
+    c1 = oc.encrypt(bytes([74]) * 32, pk, m1)
+    c2 = oc.encrypt(bytes([75]) * 32, pk, m2, entry=1)
+    qv = oc.mod[oc.oQ:oc.oQ + oc.qn].reshape(1, oc.qn, 1)
+    ops = {
+        "encrypt": lambda: oc.encrypt(bytes([76]) * 32, pk, m1),
+        "add": lambda: (c1 + c2) % qv,
+        "mult_relin": lambda: oc.mult_relin(rlk, c1, c2),
+        "decrypt": lambda: oc.decrypt(sk, c1),
+    }
+    # the product must decrypt to the slot-wise product
+    dec = oc.decode(oc.decrypt(sk, oc.mult_relin(rlk, c1, c2)))
+    s1, s2 = oc.decode(m1), oc.decode(m2)
+    ok = bool(np.array_equal(dec, (s1.astype(object) * s2.astype(object) % d["plain"]).astype(np.uint64)))
+
+    def rate(fn, secs):
+        fn()
+        k, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            k += 1
+            dt = time.perf_counter() - t0
+            if dt >= secs:
+                return k / dt
+
+    single = {name: round(rate(fn, seconds), 2) for name, fn in ops.items()}
+    multi = {}
+    with ThreadPoolExecutor(threads) as ex:
+        for name, fn in ops.items():
+            if name == "add":  # numpy element-wise: holds the interpreter lock, not threaded
+                continue
+            multi[name] = round(sum(ex.map(lambda _: rate(fn, seconds), range(threads))), 2)
+    return {"workload": "BFV at N=4096, Q = 2 x 36-bit primes, P = 1 x 37-bit prime, t=65537 (batching), "
+                        "seeded uniform slots, ternary secret, CBD error (sigma 3.24)",
+            "unit": "op/s", "kind": "port", "single_core": single, "threads": threads,
+            "all_threads": multi, "mult_decrypts_to_slot_product": ok,
+            "sample": f"each op repeated for {seconds:.1f} s on one core, then on {threads} threads"}
 
 
 def main():
@@ -240,6 +305,7 @@ def main():
         result["mixed_ext_product"] = mixed_c4(ctx, sk, rlk, ct1, ct2, out, ws, m1, stream)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, ct1, ct2, tuple(rlk.shape))
+        result["cpu_baseline"]["c1_n4096"] = cpu_c1(host_threads())
     del ws, out, ct1, ct2, rlk
     if not args.no_ntt_alone:
         result["ntt_n16384_l8_b256"] = ntt_alone(stream)

@@ -263,6 +263,46 @@ int kbfv_mult_relin(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t
                     const uint64_t *ct2, uint64_t *out, int batch, void *workspace,
                     size_t ws_bytes, void *stream);
 
+/* Levels.  A ciphertext at level l (0 <= l <= qn - 1) lives modulo
+ * Q_l = q_0 ... q_l and is stored compactly as [2][l + 1][N] (NTT form), the
+ * reference's Ciphertext::l (include/Ciphertext.cuh:5-30) without its unused
+ * limbs.  kbfv_context_level returns a view of the context at level l, owned
+ * by the context (never destroyed by the caller; the full level returns the
+ * context itself).  On a view, ciphertext arguments are level-l batches
+ * [batch][2][l + 1][N] and these entry points run at that level: kbfv_ntt,
+ * the element-wise and HPS primitives, kbfv_keygen_pk (a level-l public key
+ * from the full-level secret key), kbfv_encrypt (with such a key),
+ * kbfv_decrypt, kbfv_add, kbfv_negate, kbfv_mul_scalar, kbfv_add_plain and
+ * kbfv_mult_relin / kbfv_mult_relin_noalloc with the full-level relinearisation
+ * key.  Key generation, rotation and the external product need the full-level
+ * context.  HPS at level l uses R_l = the l + 2 moduli that follow q_l in the
+ * context's [Q | R] modulus table (so Q_l R_l stays one contiguous range); key
+ * switching reads the full-level key on its first ceil((l + 1) / pn) digits and
+ * ceil((pn + l + 1) / gamma) blocks, as the reference's key switch does with
+ * its swk_blockNum (include/KeySwitch_23.cuh:255-297). */
+int kbfv_context_level(const kbfv_context *ctx, int level, const kbfv_context **view);
+
+/* Modulus switching (replaces BFVScheme::rescaleToAndEqual,
+ * include/BFVScheme.cuh:469-487): level from_level -> to_level, dropping the
+ * top limbs one at a time, x_i <- (x_i - x_l) q_l^-1 mod q_i in coefficient
+ * form.  in [batch][2][from_level + 1][N], out [batch][2][to_level + 1][N],
+ * NTT form both; out must not overlap in. */
+size_t kbfv_mod_switch_workspace_bytes(const kbfv_context *ctx, int from_level, int batch);
+int kbfv_mod_switch(const kbfv_context *ctx, const uint64_t *in, int from_level, uint64_t *out, int to_level,
+                    int batch, void *workspace, size_t ws_bytes, void *stream);
+
+/* Batched HomMult + relinearisation over ciphertexts of uneven RNS length:
+ * entry b is a pair at level levels[b] (host array, 1 <= level <= qn - 1), the
+ * entries packed back to back in ct1, ct2 and out (entry b at the sum of
+ * 2 (levels[i] + 1) N words over i < b).  Each run of consecutive entries of
+ * one level is one batched pipeline at that level (sort by level for the
+ * largest batches); the result of every entry equals kbfv_mult_relin on its
+ * level's view.  The workspace may be any size that holds one full-level
+ * ciphertext (kbfv_mult_workspace_bytes(ctx, 1)); runs are chunked to fit. */
+int kbfv_mult_relin_ragged(const kbfv_context *ctx, const uint64_t *rlk, const uint64_t *ct1,
+                           const uint64_t *ct2, uint64_t *out, const int *levels, int batch, void *workspace,
+                           size_t ws_bytes, void *stream);
+
 /* Allocation-free HomMult + relinearisation.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same

@@ -76,6 +76,10 @@ SIGNATURES = {
     "kbfv_mult_workspace_bytes": (_sz, [_vp, C.c_int]),
     "kbfv_mult_relin": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _sz, _vp]),
     "kbfv_mult_noalloc_max_batch": (_sz, [_vp]),
+    "kbfv_context_level": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "kbfv_mod_switch_workspace_bytes": (_sz, [_vp, C.c_int, C.c_int]),
+    "kbfv_mod_switch": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _sz, _vp]),
+    "kbfv_mult_relin_ragged": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _ip, C.c_int, _vp, _sz, _vp]),
     "kbfv_mult_relin_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_negate": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
     "kbfv_mul_scalar": (C.c_int, [_vp, _vp, _u64, _vp, _vp, C.c_int, _vp]),
@@ -274,6 +278,35 @@ class Context:
         lib.kbfv_context_moduli(self.h, mods.ctypes.data_as(_u64p))
         self.mod = mods
         self._ws = None
+
+    def level(self, level):
+        """The context at ciphertext level `level` (Q_l = the first level + 1
+        Q primes): a view owned by this context; the full level is `self`."""
+        if level == self.qn - 1 and getattr(self, "parent", None) is None:
+            return self
+        return LevelView(self, level)
+
+    def mod_switch(self, ct, from_level, to_level, out=None, stream=None):
+        """Level from_level -> to_level (compact [batch][2][l + 1][N] batches)."""
+        batch = ct.shape[0]
+        out = out if out is not None else self.empty(batch, 2, to_level + 1, self.n)
+        ws = self.workspace(lib.kbfv_mod_switch_workspace_bytes(self.h, from_level, batch))
+        check(lib.kbfv_mod_switch(self.h, ptr(ct), from_level, ptr(out), to_level, batch, ptr(ws), ws.numel(),
+                                  stream_ptr(stream)))
+        return out
+
+    def mult_relin_ragged(self, rlk, c1, c2, levels, out=None, ws=None, stream=None):
+        """HomMult + relinearisation of pairs at the given levels, packed back to
+        back in the flat buffers c1, c2 (entry b: 2 (levels[b] + 1) N words)."""
+        need = sum(2 * (int(lvl) + 1) * self.n for lvl in levels)
+        assert c1.numel() == need and c2.numel() == need, "packed batch size does not match the levels"
+        lv = (C.c_int * len(levels))(*levels)
+        out = out if out is not None else self.empty(*c1.shape)
+        assert out.numel() == need
+        ws = ws if ws is not None else self.workspace(self.mult_workspace_bytes(1))
+        check(lib.kbfv_mult_relin_ragged(self.h, ptr(rlk), ptr(c1), ptr(c2), ptr(out), lv, len(levels), ptr(ws),
+                                         ws.numel(), stream_ptr(stream)))
+        return out
 
     @classmethod
     def default(cls, log_n):
@@ -690,6 +723,29 @@ def _cache_read(path):
 
 
 _BUILD_ID = None
+
+
+class LevelView(Context):
+    """A Context at one ciphertext level below the full one (kbfv_context_level):
+    same methods, ciphertexts [batch][2][level + 1][N]; owned by its parent."""
+
+    def __init__(self, parent, level):
+        h = C.c_void_p()
+        check(lib.kbfv_context_level(parent.h, level, C.byref(h)))
+        self.h = h
+        self.parent = parent  # keeps the owning context alive
+        info = (C.c_int * 16)()
+        check(lib.kbfv_context_info(self.h, info))
+        for k, v in zip(self.INFO, info):
+            setattr(self, k, v)
+        self.plain = parent.plain
+        self.mod = parent.mod
+        self.level_index = level
+        self._q, self._p, self._t = parent._q, parent._p, parent._t
+        self._ws = None
+
+    def close(self):
+        self.h = None  # the parent owns the view
 
 
 def library_build_id():

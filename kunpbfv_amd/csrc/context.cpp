@@ -299,6 +299,198 @@ std::vector<kbfv_tw> tws(const std::vector<uint64_t> &v, const std::vector<uint6
     return r;
 }
 
+// int8-MFMA operand tables.  For a K x J matrix M mod p_j: the expanded matrix
+// [JP][8][KP] of packed digit bytes (entry E[j][v][i] packs, in byte u, digit
+// v of the balanced digits of 2^(8u) M_ij mod p_j), padded to 16 ceil(J/16)
+// rows and 8 ceil(K/8) columns with zeros; then per output row j KB_MX_ROW
+// words: the offset (-KB_MX_ACC_BIAS * sum_v 2^(8v)) mod p_j that cancels the
+// accumulator bias, 2^64 - p_j, the offset and 1/p_j as doubles, and the
+// diagonal coefficient of scale-and-round with its Shoup companion.
+template <class Entry, class Diag>
+std::vector<uint64_t> mfma_table(int K, int J, const std::vector<uint64_t> &pj, Entry entry, Diag diag) {
+    const int KP = 8 * ((K + 7) / 8), JP = 16 * ((J + 15) / 16);
+    std::vector<uint64_t> v((size_t)JP * 8 * KP + KB_MX_ROW * (size_t)JP, 0);
+    kb_u128_host bias_sum = 0;
+    for (int u = 0; u < 8; u++) bias_sum += (kb_u128_host)KB_MX_ACC_BIAS << (8 * u);
+    for (int j = 0; j < J; j++) {
+        for (int i = 0; i < K; i++) {
+            const uint64_t m = entry(i, j) % pj[j];
+            for (int u = 0; u < 8; u++) {
+                const uint64_t mu = (uint64_t)(((kb_u128_host)m << (8 * u)) % pj[j]);
+                const uint64_t d = (mu + KB_DIGIT_BIAS) ^ KB_DIGIT_BIAS;
+                for (int w = 0; w < 8; w++) v[((size_t)j * 8 + w) * KP + i] |= ((d >> (8 * w)) & 0xff) << (8 * u);
+            }
+        }
+        uint64_t *row = v.data() + (size_t)JP * 8 * KP + KB_MX_ROW * (size_t)j;
+        const uint64_t off = (pj[j] - (uint64_t)(bias_sum % pj[j])) % pj[j];
+        const double offd = (double)off, invp = 1.0 / (double)pj[j];
+        const uint64_t dg = diag(j) % pj[j];
+        row[0] = off;
+        row[1] = 0 - pj[j];
+        std::memcpy(&row[2], &offd, 8);
+        std::memcpy(&row[3], &invp, 8);
+        row[4] = dg;
+        row[5] = shoup(dg, pj[j]);
+    }
+    return v;
+}
+
+// The tables that depend on the ciphertext modulus Q (and the HPS basis R
+// that goes with it): HPS base conversion Q <-> R and scale-and-round
+// (include/BasisConv_QR.cuh:53-189, include/BFVcontext.cuh:1268-1388), HPS
+// decryption (:1024-1137), the encryption constant -Q mod t (:881-924) and the
+// int8-MFMA operands of the first two.  Built once for the full Q and once
+// per level view (Q_l, R_l).
+struct LevelTables {
+    std::vector<uint64_t> q_hatinv, q_hat_mod_r, alphaQ, r_hatinv, r_hat_mod_q, alphaR, sr_modr;
+    std::vector<double> q_inv, r_inv, sr_frac, dec_frac, dec_B_frac;
+    std::vector<uint64_t> dec_modt, dec_B_modt, q_hatinv_n, mx_qr, mx_rq, mx_sr;
+    int qMSB = 0, sizeQMSB = 0, tMSB = 0;
+    uint64_t negQ_mod_t = 0;
+    bool mx_ok = false;
+};
+
+LevelTables level_tables(const std::vector<uint64_t> &Q, const std::vector<uint64_t> &R, uint64_t plain, int N) {
+    const int qn = (int)Q.size(), rn = (int)R.size();
+    LevelTables L;
+    // HPS base converters (include/BasisConv_QR.cuh:53-189)
+    L.q_hatinv.resize(qn), L.q_hat_mod_r.resize((size_t)rn * qn), L.alphaQ.resize((size_t)(qn + 1) * rn);
+    L.q_inv.resize(qn), L.r_inv.resize(rn);
+    L.r_hatinv.resize(rn), L.r_hat_mod_q.resize((size_t)qn * rn), L.alphaR.resize((size_t)(rn + 1) * qn);
+    for (int i = 0; i < qn; i++) {
+        L.q_hatinv[i] = invmod(prod_except(Q, 0, qn, i, Q[i]), Q[i]);
+        L.q_inv[i] = 1.0 / (double)Q[i];
+    }
+    for (int j = 0; j < rn; j++) {
+        for (int i = 0; i < qn; i++) L.q_hat_mod_r[j * qn + i] = prod_except(Q, 0, qn, i, R[j]);
+        uint64_t Qm = prod_except(Q, 0, qn, -1, R[j]);
+        for (int a = 0; a <= qn; a++) L.alphaQ[a * rn + j] = mulmod(a, Qm, R[j]);
+    }
+    for (int i = 0; i < rn; i++) {
+        L.r_hatinv[i] = invmod(prod_except(R, 0, rn, i, R[i]), R[i]);
+        L.r_inv[i] = 1.0 / (double)R[i];
+    }
+    for (int j = 0; j < qn; j++) {
+        for (int i = 0; i < rn; i++) L.r_hat_mod_q[j * rn + i] = prod_except(R, 0, rn, i, Q[j]);
+        uint64_t Rm = prod_except(R, 0, rn, -1, Q[j]);
+        for (int a = 0; a <= rn; a++) L.alphaR[a * qn + j] = mulmod(a, Rm, Q[j]);
+    }
+    // HPS scale-and-round tables (include/BFVcontext.cuh:1268-1388)
+    L.sr_modr.assign((size_t)rn * (qn + 1), 0);
+    L.sr_frac.assign(qn, 0);
+    {
+        std::vector<uint64_t> S(Q);
+        S.insert(S.end(), R.begin(), R.end());
+        int sn = qn + rn;
+        Big tR(plain);
+        for (uint64_t r : R) tR.mul(r);
+        for (int i = 0; i < sn; i++) {
+            uint64_t shatinv = invmod(prod_except(S, 0, sn, i, S[i]), S[i]);
+            Big x = tR;
+            x.mul(shatinv);
+            if (i < qn) L.sr_frac[i] = (double)x.mod(S[i]) / (double)S[i];
+            x.divmod(S[i]);
+            if (i < qn)
+                for (int j = 0; j < rn; j++) L.sr_modr[j * (qn + 1) + i] = x.mod(R[j]);
+            else
+                L.sr_modr[(i - qn) * (qn + 1) + qn] = x.mod(R[i - qn]);
+        }
+    }
+    // HPS decrypt (include/BFVcontext.cuh:1024-1137)
+    L.dec_modt.resize(qn), L.dec_B_modt.resize(qn), L.dec_frac.resize(qn), L.dec_B_frac.resize(qn);
+    {
+        uint64_t qmax = *std::max_element(Q.begin(), Q.end());
+        L.qMSB = bits_of(qmax), L.sizeQMSB = bits_of((uint64_t)qn), L.tMSB = bits_of(plain);
+        int hf = L.qMSB >> 1;
+        for (int i = 0; i < qn; i++) {
+            kb_u128_host x = (kb_u128_host)plain * L.q_hatinv[i];
+            L.dec_modt[i] = (uint64_t)((x / Q[i]) % plain);
+            L.dec_frac[i] = (double)(uint64_t)(x % Q[i]) / (double)Q[i];
+            uint64_t hb = (uint64_t)(((kb_u128_host)L.q_hatinv[i] << hf) % Q[i]);
+            kb_u128_host y = (kb_u128_host)plain * hb;
+            L.dec_B_modt[i] = (uint64_t)((y / Q[i]) % plain);
+            L.dec_B_frac[i] = (double)(uint64_t)(y % Q[i]) / (double)Q[i];
+        }
+    }
+    // encrypt (include/BFVcontext.cuh:881-924)
+    L.negQ_mod_t = plain - prod_except(Q, 0, qn, -1, plain);
+    // [q_hatinv n^-1]_q for inputs that are N times the INTT
+    L.q_hatinv_n.resize(qn);
+    for (int i = 0; i < qn; i++) L.q_hatinv_n[i] = mulmod(L.q_hatinv[i], invmod((uint64_t)N % Q[i], Q[i]), Q[i]);
+    // every matrix of the MFMA path has K, J <= 32, and its epilogue's quotient
+    // estimate needs output primes of at least 2^58
+    L.mx_ok = rn + 1 <= 32;
+    for (uint64_t m : Q) L.mx_ok = L.mx_ok && m >= (1ULL << 58);
+    for (uint64_t m : R) L.mx_ok = L.mx_ok && m >= (1ULL << 58);
+    if (L.mx_ok) {
+        auto none = [](int) { return (uint64_t)0; };
+        L.mx_qr = mfma_table(
+            qn + 1, rn, R,
+            [&](int i, int j) { return i < qn ? L.q_hat_mod_r[j * qn + i] : (R[j] - L.alphaQ[rn + j]) % R[j]; },
+            none);
+        L.mx_rq = mfma_table(
+            rn + 1, qn, Q,
+            [&](int i, int j) { return i < rn ? L.r_hat_mod_q[j * rn + i] : (Q[j] - L.alphaR[qn + j]) % Q[j]; },
+            none);
+        L.mx_sr = mfma_table(
+            qn, rn, R, [&](int i, int j) { return L.sr_modr[j * (qn + 1) + i]; },
+            [&](int j) { return L.sr_modr[j * (qn + 1) + qn]; });
+    }
+    return L;
+}
+
+struct LevelOffsets {
+    size_t qhi, qhr, aq, qinv, rhi, rhq, ar, rinv, srm, srf, dm, df, dbm, dbf, qhin, mxqr, mxrq, mxsr;
+};
+
+LevelOffsets add_level(Blob &b, const LevelTables &L, const std::vector<uint64_t> &Q,
+                       const std::vector<uint64_t> &R, uint64_t plain) {
+    const std::vector<uint64_t> tvec_plain(Q.size(), plain);
+    LevelOffsets o;
+    o.qhi = b.add(tws(L.q_hatinv, Q)), o.qhr = b.add(L.q_hat_mod_r), o.aq = b.add(L.alphaQ);
+    o.qinv = b.add(L.q_inv);
+    o.rhi = b.add(tws(L.r_hatinv, R)), o.rhq = b.add(L.r_hat_mod_q), o.ar = b.add(L.alphaR);
+    o.rinv = b.add(L.r_inv);
+    o.srm = b.add(L.sr_modr), o.srf = b.add(L.sr_frac);
+    o.dm = b.add(tws(L.dec_modt, tvec_plain)), o.df = b.add(L.dec_frac);
+    o.dbm = b.add(tws(L.dec_B_modt, tvec_plain)), o.dbf = b.add(L.dec_B_frac);
+    o.qhin = b.add(tws(L.q_hatinv_n, Q));
+    o.mxqr = b.add(L.mx_qr), o.mxrq = b.add(L.mx_rq), o.mxsr = b.add(L.mx_sr);
+    return o;
+}
+
+void bind_level(kb_dev_tables &d, const uint8_t *base, const LevelOffsets &o, bool mx_ok) {
+    auto at = [&](size_t off) { return (const void *)(base + off); };
+    d.q_hatinv = (const kbfv_tw *)at(o.qhi), d.q_hat_mod_r = (const uint64_t *)at(o.qhr);
+    d.alphaQ_mod_r = (const uint64_t *)at(o.aq), d.q_inv = (const double *)at(o.qinv);
+    d.r_hatinv = (const kbfv_tw *)at(o.rhi), d.r_hat_mod_q = (const uint64_t *)at(o.rhq);
+    d.alphaR_mod_q = (const uint64_t *)at(o.ar), d.r_inv = (const double *)at(o.rinv);
+    d.sr_modr = (const uint64_t *)at(o.srm), d.sr_frac = (const double *)at(o.srf);
+    d.dec_modt = (const kbfv_tw *)at(o.dm), d.dec_frac = (const double *)at(o.df);
+    d.dec_B_modt = (const kbfv_tw *)at(o.dbm), d.dec_B_frac = (const double *)at(o.dbf);
+    d.q_hatinv_n = (const kbfv_tw *)at(o.qhin);
+    d.mx_qr = mx_ok ? (const uint64_t *)at(o.mxqr) : nullptr;
+    d.mx_rq = mx_ok ? (const uint64_t *)at(o.mxrq) : nullptr;
+    d.mx_sr = mx_ok ? (const uint64_t *)at(o.mxsr) : nullptr;
+}
+
+void set_level_scalars(kbfv_context *c, const LevelTables &L, uint64_t plain) {
+    c->qMSB = L.qMSB, c->sizeQMSB = L.sizeQMSB, c->tMSB = L.tMSB;
+    c->negQ_mod_t = L.negQ_mod_t;
+    c->negQ_mod_t_shoup = shoup(L.negQ_mod_t, plain);
+}
+
+void free_context(kbfv_context *c) {
+    for (kbfv_context *v : c->views) {
+        if (v->d_level_blob) (void)hipFree(v->d_level_blob);
+        delete v;
+    }
+    c->views.clear();
+    if (c->d_blob) (void)hipFree(c->d_blob);
+    if (c->d_tw) (void)hipFree(c->d_tw);
+    delete c;
+}
+
 }  // namespace
 
 extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, int pn,
@@ -387,69 +579,16 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     std::vector<uint64_t> P(mod.begin() + c->oP, mod.begin() + c->oP + pn);
     std::vector<uint64_t> PQ(mod.begin(), mod.begin() + pq);
 
-    // HPS base converters (include/BasisConv_QR.cuh:53-189)
-    std::vector<uint64_t> q_hatinv(qn), q_hat_mod_r((size_t)rn * qn), alphaQ((size_t)(qn + 1) * rn);
-    std::vector<double> q_inv(qn), r_inv(rn);
-    std::vector<uint64_t> r_hatinv(rn), r_hat_mod_q((size_t)qn * rn), alphaR((size_t)(rn + 1) * qn);
-    for (int i = 0; i < qn; i++) {
-        q_hatinv[i] = invmod(prod_except(Q, 0, qn, i, Q[i]), Q[i]);
-        q_inv[i] = 1.0 / (double)Q[i];
-    }
-    for (int j = 0; j < rn; j++) {
-        for (int i = 0; i < qn; i++) q_hat_mod_r[j * qn + i] = prod_except(Q, 0, qn, i, R[j]);
-        uint64_t Qm = prod_except(Q, 0, qn, -1, R[j]);
-        for (int a = 0; a <= qn; a++) alphaQ[a * rn + j] = mulmod(a, Qm, R[j]);
-    }
-    for (int i = 0; i < rn; i++) {
-        r_hatinv[i] = invmod(prod_except(R, 0, rn, i, R[i]), R[i]);
-        r_inv[i] = 1.0 / (double)R[i];
-    }
-    for (int j = 0; j < qn; j++) {
-        for (int i = 0; i < rn; i++) r_hat_mod_q[j * rn + i] = prod_except(R, 0, rn, i, Q[j]);
-        uint64_t Rm = prod_except(R, 0, rn, -1, Q[j]);
-        for (int a = 0; a <= rn; a++) alphaR[a * qn + j] = mulmod(a, Rm, Q[j]);
-    }
-    // HPS scale-and-round tables (include/BFVcontext.cuh:1268-1388)
-    std::vector<uint64_t> sr_modr((size_t)rn * (qn + 1));
-    std::vector<double> sr_frac(qn);
-    {
-        std::vector<uint64_t> S(Q);
-        S.insert(S.end(), R.begin(), R.end());
-        int sn = qn + rn;
-        Big tR(plain);
-        for (uint64_t r : R) tR.mul(r);
-        for (int i = 0; i < sn; i++) {
-            uint64_t shatinv = invmod(prod_except(S, 0, sn, i, S[i]), S[i]);
-            Big x = tR;
-            x.mul(shatinv);
-            if (i < qn) sr_frac[i] = (double)x.mod(S[i]) / (double)S[i];
-            x.divmod(S[i]);
-            if (i < qn)
-                for (int j = 0; j < rn; j++) sr_modr[j * (qn + 1) + i] = x.mod(R[j]);
-            else
-                sr_modr[(i - qn) * (qn + 1) + qn] = x.mod(R[i - qn]);
-        }
-    }
-    // HPS decrypt (include/BFVcontext.cuh:1024-1137)
-    std::vector<uint64_t> dec_modt(qn), dec_B_modt(qn);
-    std::vector<double> dec_frac(qn), dec_B_frac(qn);
-    {
-        uint64_t qmax = *std::max_element(Q.begin(), Q.end());
-        c->qMSB = bits_of(qmax), c->sizeQMSB = bits_of((uint64_t)qn), c->tMSB = bits_of(plain);
-        int hf = c->qMSB >> 1;
-        for (int i = 0; i < qn; i++) {
-            kb_u128_host x = (kb_u128_host)plain * q_hatinv[i];
-            dec_modt[i] = (uint64_t)((x / Q[i]) % plain);
-            dec_frac[i] = (double)(uint64_t)(x % Q[i]) / (double)Q[i];
-            uint64_t hb = (uint64_t)(((kb_u128_host)q_hatinv[i] << hf) % Q[i]);
-            kb_u128_host y = (kb_u128_host)plain * hb;
-            dec_B_modt[i] = (uint64_t)((y / Q[i]) % plain);
-            dec_B_frac[i] = (double)(uint64_t)(y % Q[i]) / (double)Q[i];
-        }
-    }
-    // encrypt (include/BFVcontext.cuh:881-924)
-    c->negQ_mod_t = plain - prod_except(Q, 0, qn, -1, plain);
-    c->negQ_mod_t_shoup = shoup(c->negQ_mod_t, plain);
+    // HPS base conversion, scale-and-round, decryption and encryption tables
+    // of the full level
+    const LevelTables L0 = level_tables(Q, R, plain, N);
+    const std::vector<uint64_t> &q_hatinv = L0.q_hatinv, &q_hat_mod_r = L0.q_hat_mod_r, &alphaQ = L0.alphaQ;
+    const std::vector<uint64_t> &r_hatinv = L0.r_hatinv, &r_hat_mod_q = L0.r_hat_mod_q, &alphaR = L0.alphaR;
+    const std::vector<uint64_t> &sr_modr = L0.sr_modr, &dec_modt = L0.dec_modt, &dec_B_modt = L0.dec_B_modt;
+    const std::vector<double> &q_inv = L0.q_inv, &r_inv = L0.r_inv, &sr_frac = L0.sr_frac;
+    const std::vector<double> &dec_frac = L0.dec_frac, &dec_B_frac = L0.dec_B_frac;
+    set_level_scalars(c, L0, plain);
+    c->level = qn - 1;
     std::vector<uint64_t> tinv(qn);
     for (int i = 0; i < qn; i++) tinv[i] = invmod(plain % Q[i], Q[i]);
     // key switching (include/BFVcontext.cuh:286-490)
@@ -493,8 +632,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     }
     // the same constants for inputs that are N times the INTT (its n^-1 left
     // out): [q_hatinv n^-1]_q, [t_hatinv n^-1]_t and [N halfT]_t
-    std::vector<uint64_t> q_hatinv_n(qn), t_hatinv_n(tn), halfT_n(tn);
-    for (int i = 0; i < qn; i++) q_hatinv_n[i] = mulmod(q_hatinv[i], invmod((uint64_t)N % Q[i], Q[i]), Q[i]);
+    std::vector<uint64_t> t_hatinv_n(tn), halfT_n(tn);
     for (int k = 0; k < tn; k++) {
         t_hatinv_n[k] = mulmod(t_hatinv[k], invmod((uint64_t)N % T[k], T[k]), T[k]);
         halfT_n[k] = mulmod(halfT[pq + k], (uint64_t)N % T[k], T[k]);
@@ -543,61 +681,6 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     keepf("q_inv", q_inv), keepf("r_inv", r_inv), keepf("sr_frac", sr_frac);
     keepf("dec_frac", dec_frac), keepf("dec_B_frac", dec_B_frac);
 
-    // int8-MFMA operand tables.  For a K x J matrix M mod p_j: entry
-    // E[j][v][i] packs, in byte u, digit v of the balanced digits of
-    // (2^(8u) M_ij mod p_j); then per j the 128-bit offset p_j*ceil(2^76/p_j)
-    // that makes the signed digit sum non-negative.  Padded to 16 ceil(J/16)
-    // rows and 8 ceil(K/8) columns with zeros.
-    // Expanded matrix [JP][8][KP] of packed digit bytes, then per output row j
-    // KB_MX_ROW words: the offset (-KB_MX_ACC_BIAS * sum_v 2^(8v)) mod p_j that
-    // cancels the accumulator bias, 2^64 - p_j, the offset and 1/p_j as doubles,
-    // and the diagonal coefficient of scale-and-round with its Shoup companion.
-    auto mfma_table = [&](int K, int J, const std::vector<uint64_t> &pj, auto entry, auto diag) {
-        const int KP = 8 * ((K + 7) / 8), JP = 16 * ((J + 15) / 16);
-        std::vector<uint64_t> v((size_t)JP * 8 * KP + KB_MX_ROW * (size_t)JP, 0);
-        kb_u128_host bias_sum = 0;
-        for (int u = 0; u < 8; u++) bias_sum += (kb_u128_host)KB_MX_ACC_BIAS << (8 * u);
-        for (int j = 0; j < J; j++) {
-            for (int i = 0; i < K; i++) {
-                const uint64_t m = entry(i, j) % pj[j];
-                for (int u = 0; u < 8; u++) {
-                    const uint64_t mu = (uint64_t)(((kb_u128_host)m << (8 * u)) % pj[j]);
-                    const uint64_t d = (mu + KB_DIGIT_BIAS) ^ KB_DIGIT_BIAS;
-                    for (int w = 0; w < 8; w++)
-                        v[((size_t)j * 8 + w) * KP + i] |= ((d >> (8 * w)) & 0xff) << (8 * u);
-                }
-            }
-            uint64_t *row = v.data() + (size_t)JP * 8 * KP + KB_MX_ROW * (size_t)j;
-            const uint64_t off = (pj[j] - (uint64_t)(bias_sum % pj[j])) % pj[j];
-            const double offd = (double)off, invp = 1.0 / (double)pj[j];
-            const uint64_t dg = diag(j) % pj[j];
-            row[0] = off;
-            row[1] = 0 - pj[j];
-            std::memcpy(&row[2], &offd, 8);
-            std::memcpy(&row[3], &invp, 8);
-            row[4] = dg;
-            row[5] = shoup(dg, pj[j]);
-        }
-        return v;
-    };
-    // every matrix of the MFMA path has K, J <= 32, and its epilogue's quotient
-    // estimate needs output primes of at least 2^58
-    bool mx_ok = rn + 1 <= 32;
-    for (int i = 0; i < qn + rn; i++) mx_ok = mx_ok && mod[c->oQ + i] >= (1ULL << 58);  // Q then R
-    std::vector<uint64_t> mx_qr, mx_rq, mx_sr;
-    if (mx_ok) {
-        auto none = [](int) { return (uint64_t)0; };
-        mx_qr = mfma_table(
-            qn + 1, rn, R,
-            [&](int i, int j) { return i < qn ? q_hat_mod_r[j * qn + i] : (R[j] - alphaQ[rn + j]) % R[j]; }, none);
-        mx_rq = mfma_table(
-            rn + 1, qn, Q,
-            [&](int i, int j) { return i < rn ? r_hat_mod_q[j * rn + i] : (Q[j] - alphaR[qn + j]) % Q[j]; }, none);
-        mx_sr = mfma_table(
-            qn, rn, R, [&](int i, int j) { return sr_modr[j * (qn + 1) + i]; },
-            [&](int j) { return sr_modr[j * (qn + 1) + qn]; });
-    }
-
     // SEAL slot order (include/BFVcontext.cuh:102-129): row 0 slot i at the NTT
     // index of psi^(5^i), row 1 at that of psi^(-5^i)
     std::vector<uint64_t> seal_map;
@@ -613,15 +696,8 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
 
     // device upload: one blob for the small tables
     Blob blob;
-    std::vector<uint64_t> tvec_mod_q(Q), tvec_plain(qn, plain);
     size_t o_mod = blob.add(mod), o_mu = blob.add(mu), o_ninv = blob.add(ninv);
-    size_t o_qhi = blob.add(tws(q_hatinv, Q)), o_qhr = blob.add(q_hat_mod_r), o_aq = blob.add(alphaQ);
-    size_t o_qinv = blob.add(q_inv);
-    size_t o_rhi = blob.add(tws(r_hatinv, R)), o_rhq = blob.add(r_hat_mod_q), o_ar = blob.add(alphaR);
-    size_t o_rinv = blob.add(r_inv);
-    size_t o_srm = blob.add(sr_modr), o_srf = blob.add(sr_frac);
-    size_t o_dm = blob.add(tws(dec_modt, tvec_plain)), o_df = blob.add(dec_frac);
-    size_t o_dbm = blob.add(tws(dec_B_modt, tvec_plain)), o_dbf = blob.add(dec_B_frac);
+    const LevelOffsets o_lvl = add_level(blob, L0, Q, R, plain);
     size_t o_ti = blob.add(tws(tinv, Q));
     size_t o_rih = blob.add(tws(ri_hatinv, ri_hatinv_mod)), o_rht = blob.add(ri_hat_mod_t);
     size_t o_rmt = blob.add(ri_mod_t);
@@ -630,9 +706,13 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     size_t o_th = blob.add(tws(t_hatinv, T)), o_thp = blob.add(t_hat_mod_pq), o_tmp = blob.add(t_mod_pq);
     size_t o_half = blob.add(halfT), o_phq = blob.add(p_hat_mod_q), o_phi = blob.add(tws(p_hatinv, P));
     size_t o_pinv = blob.add(tws(Pinv, Q)), o_pmq = blob.add(P_mod_q);
-    size_t o_mxqr = blob.add(mx_qr), o_mxrq = blob.add(mx_rq), o_mxsr = blob.add(mx_sr);
+    // modulus switching (the reference's qiInvVecModql, include/BFVScheme.cuh:403-418)
+    std::vector<kbfv_tw> ql_inv((size_t)qn * qn, kbfv_tw{0, 0});
+    for (int l = 0; l < qn; l++)
+        for (int i = 0; i < l; i++) ql_inv[(size_t)l * qn + i] = tw_of(invmod(Q[l] % Q[i], Q[i]), Q[i]);
+    size_t o_qli = blob.add(ql_inv);
     size_t o_seal = blob.add(seal_map);
-    size_t o_qhin = blob.add(tws(q_hatinv_n, Q)), o_thn = blob.add(tws(t_hatinv_n, T)), o_hfn = blob.add(halfT_n);
+    size_t o_thn = blob.add(tws(t_hatinv_n, T)), o_hfn = blob.add(halfT_n);
     size_t o_iw1 = blob.add(inv_w1);
 
     hipError_t e = hipGetDevice(&c->device);
@@ -658,13 +738,7 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     kb_dev_tables &d = c->t;
     d.mod = (const uint64_t *)at(o_mod), d.mu = (const uint64_t *)at(o_mu);
     d.tw = c->d_tw, d.ninv = (const kbfv_tw *)at(o_ninv);
-    d.q_hatinv = (const kbfv_tw *)at(o_qhi), d.q_hat_mod_r = (const uint64_t *)at(o_qhr);
-    d.alphaQ_mod_r = (const uint64_t *)at(o_aq), d.q_inv = (const double *)at(o_qinv);
-    d.r_hatinv = (const kbfv_tw *)at(o_rhi), d.r_hat_mod_q = (const uint64_t *)at(o_rhq);
-    d.alphaR_mod_q = (const uint64_t *)at(o_ar), d.r_inv = (const double *)at(o_rinv);
-    d.sr_modr = (const uint64_t *)at(o_srm), d.sr_frac = (const double *)at(o_srf);
-    d.dec_modt = (const kbfv_tw *)at(o_dm), d.dec_frac = (const double *)at(o_df);
-    d.dec_B_modt = (const kbfv_tw *)at(o_dbm), d.dec_B_frac = (const double *)at(o_dbf);
+    bind_level(d, (const uint8_t *)c->d_blob, o_lvl, L0.mx_ok);
     d.tinv_mod_q = (const kbfv_tw *)at(o_ti);
     d.ri_hatinv = (const kbfv_tw *)at(o_rih), d.ri_hat_mod_t = (const uint64_t *)at(o_rht);
     d.ri_mod_t = (const uint64_t *)at(o_rmt);
@@ -672,24 +746,86 @@ extern "C" int kbfv_context_create(int log_n, const uint64_t *q, int qn, const u
     d.qj_mod_t = (const uint64_t *)at(o_qjm);
     d.t_hatinv = (const kbfv_tw *)at(o_th), d.t_hat_mod_pq = (const uint64_t *)at(o_thp);
     d.t_mod_pq = (const uint64_t *)at(o_tmp), d.halfT_mod = (const uint64_t *)at(o_half);
+    d.halfT_t = d.halfT_mod + pq;
     d.p_hat_mod_q = (const uint64_t *)at(o_phq), d.p_hatinv = (const kbfv_tw *)at(o_phi);
     d.Pinv_mod_q = (const kbfv_tw *)at(o_pinv), d.P_mod_q = (const uint64_t *)at(o_pmq);
+    d.ql_inv_mod_qi = (const kbfv_tw *)at(o_qli);
     if (c->batching) d.seal_map = (const uint64_t *)at(o_seal);
-    d.q_hatinv_n = (const kbfv_tw *)at(o_qhin), d.t_hatinv_n = (const kbfv_tw *)at(o_thn);
+    d.t_hatinv_n = (const kbfv_tw *)at(o_thn);
     d.halfT_n = (const uint64_t *)at(o_hfn), d.inv_w1 = (const kbfv_tw *)at(o_iw1);
-    if (mx_ok) {
-        d.mx_qr = (const uint64_t *)at(o_mxqr), d.mx_rq = (const uint64_t *)at(o_mxrq);
-        d.mx_sr = (const uint64_t *)at(o_mxsr);
+    c->key_dnum = c->dnum, c->key_blocks = c->ri_blocks;
+
+    // level views l = 0 .. qn - 2: Q_l = q_0..q_l and R_l = the l + 2 moduli
+    // that follow q_l in the [Q | R] table, so that Q_l R_l is again one
+    // contiguous range of the table (the HomMult pipeline transforms and
+    // tensors over it as one range); keys of the full level are read on their
+    // first dnum_l digits and ri_blocks_l blocks (the last block's limbs past
+    // Q_l are produced and dropped)
+    for (int l = 0; l + 1 < qn; ++l) {
+        const std::vector<uint64_t> Ql(mod.begin() + c->oQ, mod.begin() + c->oQ + l + 1),
+            Rl(mod.begin() + c->oQ + l + 1, mod.begin() + c->oQ + 2 * l + 3);
+        const LevelTables Ll = level_tables(Ql, Rl, plain, N);
+        Blob lb;
+        const LevelOffsets ol = add_level(lb, Ll, Ql, Rl, plain);
+        // digit groups of pn primes over Q_l (with pn > 1 the last one may be
+        // shorter than at the full level)
+        const int Dl = (l + 1 + pn - 1) / pn;
+        std::vector<uint64_t> lqh((size_t)Dl * pn), lqh_mod((size_t)Dl * pn, 1), lqht((size_t)Dl * tn * pn),
+            lqm((size_t)Dl * tn);
+        for (int b = 0; b < Dl; b++) {
+            int lo = b * pn, hi = std::min(lo + pn, l + 1);
+            for (int j = lo; j < hi; j++) {
+                lqh[b * pn + j - lo] = invmod(prod_except(Ql, lo, hi, j, Ql[j]), Ql[j]);
+                lqh_mod[b * pn + j - lo] = Ql[j];
+                for (int k = 0; k < tn; k++) lqht[(b * tn + k) * pn + j - lo] = prod_except(Ql, lo, hi, j, T[k]);
+            }
+            for (int k = 0; k < tn; k++) lqm[b * tn + k] = prod_except(Ql, lo, hi, -1, T[k]);
+        }
+        const size_t o_lqh = lb.add(tws(lqh, lqh_mod)), o_lqht = lb.add(lqht), o_lqm = lb.add(lqm);
+        auto v = new kbfv_context();
+        c->views.push_back(v);
+        v->parent = c, v->level = l;
+        v->log_n = c->log_n, v->n = c->n;
+        v->qn = l + 1, v->pn = pn, v->tn = tn, v->rn = l + 2, v->gamma = gamma;
+        v->dnum = (l + 1 + pn - 1) / pn;
+        v->ri_blocks = (pn + l + 1 + gamma - 1) / gamma;
+        v->key_dnum = c->dnum, v->key_blocks = c->ri_blocks;
+        v->batching = c->batching, v->nmod = c->nmod;
+        v->oP = c->oP, v->oQ = c->oQ, v->oR = c->oQ + l + 1, v->oT = c->oT, v->oPlain = c->oPlain;
+        v->plain = plain, v->device = c->device, v->digits_direct_t = c->digits_direct_t;
+        v->mod = c->mod;
+        set_level_scalars(v, Ll, plain);
+        v->t = c->t;
+        hipError_t ev = hipMalloc(&v->d_level_blob, lb.bytes.size());
+        if (ev == hipSuccess)
+            ev = hipMemcpy(v->d_level_blob, lb.bytes.data(), lb.bytes.size(), hipMemcpyHostToDevice);
+        if (ev != hipSuccess) {
+            kb_set_error("level tables upload: %s", hipGetErrorString(ev));
+            free_context(c);
+            return KBFV_EHIP;
+        }
+        bind_level(v->t, (const uint8_t *)v->d_level_blob, ol, Ll.mx_ok);
+        const uint8_t *vb = (const uint8_t *)v->d_level_blob;
+        v->t.qj_hatinv = (const kbfv_tw *)(vb + o_lqh);
+        v->t.qj_hat_mod_t = (const uint64_t *)(vb + o_lqht);
+        v->t.qj_mod_t = (const uint64_t *)(vb + o_lqm);
     }
     *out = c;
     return KBFV_OK;
 }
 
+extern "C" int kbfv_context_level(const kbfv_context *c, int level, const kbfv_context **view) {
+    KB_REQUIRE(c && view, "null argument");
+    const kbfv_context *root = kb_root(c);
+    KB_REQUIRE(level >= 0 && level < root->qn, "level %d out of range [0, %d]", level, root->qn - 1);
+    *view = level == root->qn - 1 ? root : root->views[level];
+    return KBFV_OK;
+}
+
 extern "C" int kbfv_context_destroy(kbfv_context *c) {
     if (!c) return KBFV_OK;
-    if (c->d_blob) (void)hipFree(c->d_blob);
-    if (c->d_tw) (void)hipFree(c->d_tw);
-    delete c;
+    KB_REQUIRE(!c->parent, "a level view belongs to its context and is destroyed with it");
+    free_context(c);
     return KBFV_OK;
 }
 

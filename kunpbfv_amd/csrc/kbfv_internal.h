@@ -82,10 +82,13 @@ struct kb_dev_tables {
     const uint64_t *t_hat_mod_pq;    // [pn+qn][tn]
     const uint64_t *t_mod_pq;        // [pn+qn]
     const uint64_t *halfT_mod;       // [pn+qn+tn] in the order P, Q, T
+    const uint64_t *halfT_t;         // [tn] = halfT_mod + pn + qn (the T part, at any level)
     const uint64_t *p_hat_mod_q;     // [qn][pn]
     const kbfv_tw *p_hatinv;         // [pn]
     const kbfv_tw *Pinv_mod_q;       // [qn]
     const uint64_t *P_mod_q;         // [qn]
+    // modulus switching: [l][i] = q_l^-1 mod q_i (with Shoup companion), i < l
+    const kbfv_tw *ql_inv_mod_qi;    // [qn][qn]
     // int8-MFMA operand tables: [16*ceil(J/16)][8][8*ceil(K/8)] packed digits of
     // 2^(8u) M_ij mod p_j, then [16*ceil(J/16)][2] offsets (null if too large)
     const uint64_t *mx_qr;  // base conversion Q -> R, K = qn + 1 (alpha row [-Q]_r)
@@ -113,6 +116,20 @@ struct kbfv_context {
     // that NTT reading the Q limb (kb_digits_to_t_ntt)
     bool digits_direct_t = false;
 
+    // level views: a context for ciphertexts at level `level` < qn_full - 1
+    // (Q_l = the first level + 1 Q primes, R_l = the level + 2 moduli after
+    // them in the [Q | R] table, oR = oQ + level + 1)
+    // shares the parent's moduli, twiddles, key-switching tables and NTT plans
+    // and has its own HPS, decryption and encryption tables.  The full-level
+    // context has parent == nullptr and level == qn - 1.
+    const kbfv_context *parent = nullptr;
+    int level = 0;
+    // shape of keys made on the full-level context ([2][key_dnum][key_blocks][tn][N]),
+    // the strides the key-switching kernels read them with at every level
+    int key_dnum = 0, key_blocks = 0;
+    std::vector<kbfv_context *> views;  // full-level context only: views[l] for l < qn - 1
+    void *d_level_blob = nullptr;       // a view's own tables
+
     // host copies (exported to parity tests)
     std::vector<uint64_t> mod, psi;
     std::vector<std::pair<std::string, std::vector<uint64_t>>> host_u64;
@@ -130,6 +147,9 @@ struct kbfv_context {
 // resolves the allocation-free HomMult's scratch address on `device` (which
 // must be current); once per device, at context creation (scheme.hip)
 int kb_noalloc_setup(int device);
+
+// the full-level context a view belongs to (the context itself otherwise)
+static inline const kbfv_context *kb_root(const kbfv_context *c) { return c->parent ? c->parent : c; }
 
 // per-limb modulus helpers
 static inline uint64_t kb_host_mulmod(uint64_t a, uint64_t b, uint64_t q) {

@@ -89,8 +89,9 @@ __global__ void k_modup_pq_to_t(kb_dev_tables t, const uint64_t *__restrict__ x,
     }
 }
 
-// Key-stationary inner product.  cin [batch][D][tn][N], key [2][D][B][tn][N],
-// out [batch][2][B][tn][N].  A workgroup owns one T limb, 64 coefficients and
+// Key-stationary inner product.  cin [batch][D][tn][N], key [2][KD][KB][tn][N]
+// (read on its first D digits and B blocks: a ciphertext below the full level
+// uses a prefix of the full-level key), out [batch][2][B][tn][N].  A workgroup owns one T limb, 64 coefficients and
 // up to IP_WAVES key blocks (one wave per block b, both key halves in
 // registers).  It walks the whole batch: per ciphertext the D x 64 digit tile
 // is staged once in LDS and read by every wave, so the key is read from HBM
@@ -99,7 +100,8 @@ constexpr int IP_WAVES = 8;
 template <int MAXD>
 __global__ void __launch_bounds__(64 * IP_WAVES)
     k_inner_product(kb_dev_tables t, const uint64_t *__restrict__ cin, const uint64_t *__restrict__ key,
-                    uint64_t *__restrict__ out, int D, int B, int tn, int oT, int n, int batch) {
+                    uint64_t *__restrict__ out, int D, int B, int KD, int KB, int tn, int oT, int n,
+                    int batch) {
     __shared__ uint64_t cs[MAXD][64];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(64 * IP_WAVES)
     const int m = oT + kk;
     const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
     const long N = n;
-    const long kstride = (long)B * tn * N;  // digit stride in the key
+    const long kstride = (long)KB * tn * N;  // digit stride in the key
     uint64_t ka[MAXD], kbv[MAXD];
 #pragma unroll
     for (int i = 0; i < MAXD; ++i) {
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(64 * IP_WAVES)
         if (active && i < D) {
             const long off = (long)i * kstride + ((long)b * tn + kk) * N + k;
             ka[i] = key[off];
-            kbv[i] = key[(long)D * kstride + off];
+            kbv[i] = key[(long)KD * kstride + off];
         }
     }
     // each thread stages at most ceil(MAXD*64 / (64*IP_WAVES)) digit values per ciphertext
@@ -168,7 +170,7 @@ __global__ void __launch_bounds__(64 * IP_WAVES)
 // fallback for dnum > 64: one thread per output, key re-read per ciphertext
 __global__ void k_inner_product_generic(kb_dev_tables t, const uint64_t *__restrict__ cin,
                                         const uint64_t *__restrict__ key, uint64_t *__restrict__ out,
-                                        int D, int B, int tn, int oT, int n, long total) {
+                                        int D, int B, int KD, int KB, int tn, int oT, int n, long total) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const long N = n;
@@ -184,7 +186,7 @@ __global__ void k_inner_product_generic(kb_dev_tables t, const uint64_t *__restr
     kbfv_u128 acc{0, 0};
     for (int d = 0; d < D; ++d)
         kb_mac128(acc, cin[((bt * D + d) * tn + kk) * N + k],
-                  key[(((long)h * D + d) * B + b) * tn * N + (long)kk * N + k]);
+                  key[(((long)h * KD + d) * KB + b) * tn * N + (long)kk * N + k]);
     out[i] = kb_barrett128(acc, t.mod[m], t.mu[2 * m], t.mu[2 * m + 1]);
 }
 
@@ -201,7 +203,7 @@ __global__ void k_modup_t_to_pq(kb_dev_tables t, const uint64_t *__restrict__ in
     float vv = 0;
     for (int kk = 0; kk < tn; ++kk) {
         const uint64_t tm = t.mod[oT + kk];
-        const uint64_t ra = ip[(long)kk * n] + t.halfT_mod[pq + kk];
+        const uint64_t ra = ip[(long)kk * n] + t.halfT_t[kk];
         const kbfv_tw h = t.t_hatinv[kk];
         rb[kk] = kb_shoup(ra, h.w, h.ws, tm);
         const float rr = (float)rb[kk];
@@ -294,7 +296,7 @@ __global__ void k_modup_moddown(kb_dev_tables t, const uint64_t *__restrict__ in
 #pragma unroll
         for (int kk = 0; kk < TN; ++kk) {
             const uint64_t tm = t.mod[oT + kk];
-            const uint64_t ra = cur[kk] + (NOSC ? t.halfT_n[kk] : t.halfT_mod[pq + kk]);
+            const uint64_t ra = cur[kk] + (NOSC ? t.halfT_n[kk] : t.halfT_t[kk]);
             const kbfv_tw hw = NOSC ? t.t_hatinv_n[kk] : t.t_hatinv[kk];
             rb[kk] = kb_shoup(ra, hw.w, hw.ws, tm);
             const float rr = (float)rb[kk];
@@ -424,7 +426,7 @@ int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t 
     const unsigned grid = (unsigned)((n / 64) * tn * ((B + IP_WAVES - 1) / IP_WAVES));
 #define KB_IP(MD)                                                                                        \
     hipLaunchKernelGGL(k_inner_product<MD>, dim3(grid), dim3(64 * IP_WAVES), 0, st, c->t, cin, key, out, D, \
-                       B, tn, c->oT, n, batch)
+                       B, c->key_dnum, c->key_blocks, tn, c->oT, n, batch)
     if (D <= 4)
         KB_IP(4);
     else if (D <= 8)
@@ -436,7 +438,7 @@ int kb_inner_product(const kbfv_context *c, const uint64_t *cin, const uint64_t 
     else {
         const long total = (long)batch * 2 * B * tn * n;
         hipLaunchKernelGGL(k_inner_product_generic, dim3(blocks_for(total, KS_BS)), dim3(KS_BS), 0, st, c->t,
-                           cin, key, out, D, B, tn, c->oT, n, total);
+                           cin, key, out, D, B, c->key_dnum, c->key_blocks, tn, c->oT, n, total);
     }
 #undef KB_IP
     KB_LAUNCH_CHECK();

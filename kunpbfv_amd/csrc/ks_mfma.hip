@@ -63,8 +63,8 @@ constexpr uint64_t IM_BIAS_HI = (uint64_t)(ip_bias_sum() >> 64);
 template <int KC, int NJ>
 __global__ void __launch_bounds__(IM_THREADS, 4)
     k_ip_mfma(const uint64_t *__restrict__ cin, const uint64_t *__restrict__ key, uint64_t *__restrict__ out,
-              const uint64_t *__restrict__ mods, const uint64_t *__restrict__ mus, int D, int B, int tn,
-              int oT, int n, int batch) {
+              const uint64_t *__restrict__ mods, const uint64_t *__restrict__ mus, int D, int B, int KD,
+              int KB, int tn, int oT, int n, int batch) {
     const int lane = threadIdx.x & 63;
     const int p16 = lane & 15, g = lane >> 4;
     const int groups = n / IM_WAVES;
@@ -81,8 +81,9 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
     const int R2 = 2 * B;
     const uint64_t q = mods[oT + kk], mlo = mus[2 * (oT + kk)], mhi = mus[2 * (oT + kk) + 1];
 
-    // key layout [2][D][B][tn][N], row r = h B + b; lane holds rows 16 nt + p16,
-    // digits d = 8c + 2g + e (zero outside 2B x D)
+    // key layout [2][KD][KB][tn][N] (its first D digits and B blocks are used),
+    // row r = h B + b; lane holds rows 16 nt + p16, digits d = 8c + 2g + e
+    // (zero outside 2B x D)
     uint64_t rk[NJ][KC][2];
     {
         const uint64_t *kb = key + kk * N + pos;
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(IM_THREADS, 4)
                     const bool live = r < R2 && d < D;
                     const int rr = live ? r : 0, dd = live ? d : 0;
                     const int h = rr / B, b = rr - h * B;
-                    const uint64_t v = kb[(size_t)(uint32_t)((h * D + dd) * B + b) * TN];
+                    const uint64_t v = kb[(size_t)(uint32_t)((h * KD + dd) * KB + b) * TN];
                     rk[nt][c][e] = __builtin_bswap64(kb_digits(live ? v : 0));
                 }
     }
@@ -217,7 +218,7 @@ int kb_inner_product_mfma(const kbfv_context *c, const uint64_t *cin, const uint
 #define KB_IM(KC, NJ)                                                                                   \
     if (kc == KC && nj == NJ) {                                                                          \
         hipLaunchKernelGGL((k_ip_mfma<KC, NJ>), grid, dim3(IM_THREADS), 0, st, cin, key, out, c->t.mod, \
-                           c->t.mu, D, B, c->tn, c->oT, c->n, batch);                                    \
+                           c->t.mu, D, B, c->key_dnum, c->key_blocks, c->tn, c->oT, c->n, batch);        \
         KB_LAUNCH_CHECK();                                                                               \
         return KBFV_OK;                                                                                  \
     }

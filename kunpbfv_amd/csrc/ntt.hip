@@ -620,7 +620,7 @@ int bucket_of(long transforms) {
 }
 
 uint32_t plan_for(const kbfv_context *c, int dir, long transforms) {
-    return c->ntt_plan[dir > 0 ? 0 : 1][bucket_of(transforms)].load(std::memory_order_relaxed);
+    return kb_root(c)->ntt_plan[dir > 0 ? 0 : 1][bucket_of(transforms)].load(std::memory_order_relaxed);
 }
 
 int check_args(const kbfv_context *c, const uint64_t *src, const uint64_t *dst, int mod0, int limbs, int polys,
@@ -822,7 +822,7 @@ extern "C" int kbfv_ntt_plan_get(const kbfv_context *c, int dir, int bucket, uin
     KB_REQUIRE(c && plan, "null argument");
     KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
     KB_REQUIRE(bucket >= 0 && bucket < KBFV_NTT_BUCKETS, "bucket out of range");
-    *plan = c->ntt_plan[dir > 0 ? 0 : 1][bucket].load(std::memory_order_relaxed);
+    *plan = kb_root(c)->ntt_plan[dir > 0 ? 0 : 1][bucket].load(std::memory_order_relaxed);
     return KBFV_OK;
 }
 
@@ -831,7 +831,8 @@ extern "C" int kbfv_ntt_plan_set(kbfv_context *c, int dir, int bucket, uint32_t 
     KB_REQUIRE(dir == 1 || dir == -1, "dir must be +1 or -1");
     KB_REQUIRE(bucket >= 0 && bucket < KBFV_NTT_BUCKETS, "bucket out of range");
     KB_REQUIRE(plan_ok(c->log_n, plan), "NTT plan 0x%x is not valid at N=2^%d", plan, c->log_n);
-    c->ntt_plan[dir > 0 ? 0 : 1][bucket].store(plan, std::memory_order_relaxed);
+    // a level view shares the plans of its full-level context
+    const_cast<kbfv_context *>(kb_root(c))->ntt_plan[dir > 0 ? 0 : 1][bucket].store(plan, std::memory_order_relaxed);
     return KBFV_OK;
 }
 
@@ -955,7 +956,8 @@ extern "C" int kbfv_ntt_autotune(kbfv_context *c, int dir, int mod0, int limbs, 
     (void)hipEventDestroy(e1);
     if (rc != KBFV_OK) return rc;
     KB_REQUIRE(best_plan != 0, "no NTT plan fits N=2^%d", log_n);
-    c->ntt_plan[dir > 0 ? 0 : 1][bucket_of((long)limbs * polys)].store(best_plan, std::memory_order_relaxed);
+    const_cast<kbfv_context *>(kb_root(c))->ntt_plan[dir > 0 ? 0 : 1][bucket_of((long)limbs * polys)].store(
+        best_plan, std::memory_order_relaxed);
     if (best) *best = best_plan;
     if (best_ms) *best_ms = best_total;
     return KBFV_OK;

@@ -373,6 +373,7 @@ extern "C" int kbfv_sample(const kbfv_context *c, int kind, const uint8_t seed[3
 
 // ---------------------------------------------------------------- keys
 extern "C" int kbfv_keygen_sk(const kbfv_context *c, const uint8_t seed[32], uint64_t *sk, void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && seed && sk, "null argument");
     const int pq = c->pn + c->qn;
     KB_TRY(kb_sample(c, 1, seed, TAG(TAG_SK), 0, c->oP, pq, sk, 0, 1, S(st)));
@@ -430,6 +431,7 @@ static int keygen_ksk(const kbfv_context *c, const uint8_t seed[32], uint64_t ta
 
 extern "C" int kbfv_keygen_rlk(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk, uint64_t *rlk,
                                void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && seed && sk && rlk && ws, "null argument");
     KB_REQUIRE(ws_bytes >= kbfv_keygen_rlk_workspace_bytes(c), "workspace too small");
     uint64_t *s2, *e, *a, *b;
@@ -441,6 +443,7 @@ extern "C" int kbfv_keygen_rlk(const kbfv_context *c, const uint8_t seed[32], co
 
 extern "C" int kbfv_keygen_swk(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk,
                                const uint64_t *src_q, uint64_t *key, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && seed && sk && src_q && key && ws, "null argument");
     KB_REQUIRE(ws_bytes >= kbfv_keygen_rlk_workspace_bytes(c), "workspace too small");
     uint64_t *s2, *e, *a, *b;
@@ -571,6 +574,71 @@ extern "C" int kbfv_mult_relin(const kbfv_context *c, const uint64_t *rlk, const
                                void *st) {
     KB_REQUIRE(c && rlk && ct1 && ct2 && out && ws && batch > 0, "null argument");
     return mult_relin_ws(c, rlk, ct1, ct2, out, batch, ws, ws_bytes, S(st));
+}
+
+// ---------------------------------------------------------------- levels
+namespace {
+// one modulus-switching step over [polys][stride][N] coefficient-form limbs:
+// x_i <- (x_i - x_l) q_l^-1 mod q_i for i < l (limb l is dropped), x_l the
+// canonical residue mod q_l
+__global__ void k_drop_limb(kb_dev_tables t, uint64_t *__restrict__ x, int l, int qn, int oQ, long stride, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    const long poly = blockIdx.z;
+    uint64_t *px = x + poly * stride;
+    const int m = oQ + i;
+    const uint64_t q = t.mod[m], mlo = t.mu[2 * m], mhi = t.mu[2 * m + 1];
+    const uint64_t xl = kb_barrett128(kbfv_u128{px[(long)l * n + k], 0}, q, mlo, mhi);
+    const uint64_t d = kb_submod(px[(long)i * n + k], xl, q);
+    const kbfv_tw w = t.ql_inv_mod_qi[(long)l * qn + i];
+    px[(long)i * n + k] = kb_shoup(d, w.w, w.ws, q);
+}
+}  // namespace
+
+extern "C" size_t kbfv_mod_switch_workspace_bytes(const kbfv_context *c, int from_level, int batch) {
+    if (!c || batch <= 0 || from_level < 0 || from_level >= kb_root(c)->qn) return 0;
+    return (size_t)batch * 2 * (from_level + 1) * c->n * sizeof(uint64_t);
+}
+
+extern "C" int kbfv_mod_switch(const kbfv_context *c, const uint64_t *in, int from_level, uint64_t *out,
+                               int to_level, int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && in && out && ws && batch > 0, "null argument");
+    const kbfv_context *r = kb_root(c);
+    KB_REQUIRE(from_level < r->qn && to_level >= 0 && to_level <= from_level, "levels %d -> %d out of range",
+               from_level, to_level);
+    KB_REQUIRE(ws_bytes >= kbfv_mod_switch_workspace_bytes(r, from_level, batch), "workspace too small");
+    KB_REQUIRE(2L * batch <= 65535, "batch too large for one launch");
+    const long N = r->n, lf = from_level + 1, lt = to_level + 1;
+    uint64_t *x = (uint64_t *)ws;
+    // coefficient form of every limb, then drop limbs from_level .. to_level + 1
+    KB_TRY(kb_ntt(r, -1, in, x, r->oQ, (int)lf, 2 * batch, 1, 0, lf * N, N, 0, lf * N, N, S(st)));
+    for (int l = from_level; l > to_level; --l) {
+        hipLaunchKernelGGL(k_drop_limb, dim3(r->n / BS, l, 2 * batch), dim3(BS), 0, S(st), r->t, x, l, r->qn, r->oQ,
+                           lf * N, r->n);
+        KB_LAUNCH_CHECK();
+    }
+    return kb_ntt(r, +1, x, out, r->oQ, (int)lt, 2 * batch, 1, 0, lf * N, N, 0, lt * N, N, S(st));
+}
+
+extern "C" int kbfv_mult_relin_ragged(const kbfv_context *c, const uint64_t *rlk, const uint64_t *ct1,
+                                      const uint64_t *ct2, uint64_t *out, const int *levels, int batch, void *ws,
+                                      size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && rlk && ct1 && ct2 && out && levels && ws && batch > 0, "null argument");
+    const kbfv_context *r = kb_root(c);
+    for (int b = 0; b < batch; ++b)
+        KB_REQUIRE(levels[b] >= 1 && levels[b] < r->qn, "entry %d: level %d out of range [1, %d]", b, levels[b],
+                   r->qn - 1);
+    // runs of equal level are contiguous compact batches: one batched pipeline each
+    long off = 0;
+    for (int b0 = 0; b0 < batch;) {
+        int b1 = b0 + 1;
+        while (b1 < batch && levels[b1] == levels[b0]) ++b1;
+        const kbfv_context *v = levels[b0] == r->qn - 1 ? r : r->views[levels[b0]];
+        KB_TRY(mult_relin_ws(v, rlk, ct1 + off, ct2 + off, out + off, b1 - b0, ws, ws_bytes, S(st)));
+        off += (long)(b1 - b0) * 2 * (levels[b0] + 1) * r->n;
+        b0 = b1;
+    }
+    return KBFV_OK;
 }
 
 extern "C" size_t kbfv_mult_noalloc_max_batch(const kbfv_context *c) {
@@ -733,6 +801,7 @@ extern "C" size_t kbfv_keygen_galois_workspace_bytes(const kbfv_context *c) {
 
 extern "C" int kbfv_keygen_galois(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk, uint64_t g,
                                   uint64_t *key, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && seed && sk && key && ws, "null argument");
     KB_REQUIRE(ws_bytes >= kbfv_keygen_galois_workspace_bytes(c), "workspace too small");
     const uint64_t m2 = 2 * (uint64_t)c->n;
@@ -782,6 +851,7 @@ extern "C" size_t kbfv_rotate_workspace_bytes(const kbfv_context *c, int batch) 
 // coefficient-form a, then the automorphism, then back to the NTT domain
 extern "C" int kbfv_rotate(const kbfv_context *c, const uint64_t *gkey, uint64_t g, const uint64_t *ct,
                            uint64_t *out, int batch, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && gkey && ct && out && ws && batch > 0, "null argument");
     int cb = batch;
     while (cb > 1 && rot_layout(c, cb, nullptr, nullptr) > ws_bytes) cb = (cb + 1) / 2;
@@ -917,6 +987,7 @@ extern "C" size_t kbfv_rgsw_encrypt_workspace_bytes(const kbfv_context *c) {
 extern "C" int kbfv_rgsw_encrypt(const kbfv_context *c, const uint8_t seed[32], const uint64_t *sk,
                                  const uint64_t *m_coeff, int base_bits, uint64_t *rgsw, void *ws, size_t ws_bytes,
                                  void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && seed && sk && m_coeff && rgsw && ws, "null argument");
     const int rows = kbfv_rgsw_rows(c, base_bits);
     KB_REQUIRE(rows > 0, "bad gadget base");
@@ -954,6 +1025,7 @@ extern "C" size_t kbfv_external_product_workspace_bytes(const kbfv_context *c, i
 
 extern "C" int kbfv_external_product(const kbfv_context *c, const uint64_t *rgsw, const uint64_t *ct, uint64_t *out,
                                      int batch, int base_bits, void *ws, size_t ws_bytes, void *st) {
+    KB_REQUIRE(c && !c->parent, "this operation needs the full-level context");
     KB_REQUIRE(c && rgsw && ct && out && ws && batch > 0, "null argument");
     const int rows = kbfv_rgsw_rows(c, base_bits);
     KB_REQUIRE(rows > 0, "bad gadget base");

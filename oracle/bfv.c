@@ -296,6 +296,42 @@ void or_relin(const or_ctx *c, const uint64_t *rlk_t, const uint64_t *d2_coeff, 
     free(pqb);
 }
 
+/* BFVScheme::rescaleToAndEqual (include/BFVScheme.cuh:469-487) with
+ * rescaleAndEqual_kernel (:403-418): inverse NTT over the from_level + 1 limbs,
+ * then for l = from_level .. to_level + 1 every limb i < l becomes
+ * (x_i - x_l) * q_l^-1 mod q_i (x_l the canonical residue mod q_l), then the
+ * forward NTT over the to_level + 1 limbs left. */
+void or_mod_switch(const or_ctx *c, const uint64_t *in, int from_level, uint64_t *out, int to_level) {
+    size_t N = c->n;
+    int lf = from_level + 1, lt = to_level + 1;
+    uint64_t *x = alloc0((size_t)2 * lf * N);
+    memcpy(x, in, (size_t)2 * lf * N * 8);
+    or_ntt_inv_range(c, x, c->oQ, lf, 2);
+    for (int h = 0; h < 2; h++) {
+        uint64_t *px = x + (size_t)h * lf * N;
+        for (int l = from_level; l > to_level; l--) {
+            uint64_t ql = c->mod[c->oQ + l];
+            for (int i = 0; i < l; i++) {
+                uint64_t qi = c->mod[c->oQ + i];
+                uint64_t inv = or_invmod(ql % qi, qi);
+                for (size_t k = 0; k < N; k++) {
+#if OR_MUTATION == 5 /* planted mistake (tests only): the centred residue of x_l (a rounding switch) */
+                    uint64_t xr = px[(size_t)l * N + k];
+                    uint64_t xl = xr > ql / 2 ? (xr % qi + qi - ql % qi) % qi : xr % qi;
+#else
+                    uint64_t xl = px[(size_t)l * N + k] % qi;
+#endif
+                    uint64_t d = (px[(size_t)i * N + k] + qi - xl) % qi;
+                    px[(size_t)i * N + k] = or_mulmod(d, inv, qi);
+                }
+            }
+        }
+        memcpy(out + (size_t)h * lt * N, px, (size_t)lt * N * 8);
+    }
+    or_ntt_fwd_range(c, out, c->oQ, lt, 2);
+    free(x);
+}
+
 void or_mult_relin(const or_ctx *c, const uint64_t *rlk_t, const uint64_t *ct1,
                    const uint64_t *ct2, uint64_t *out) {
     size_t N = c->n, L = (size_t)c->qn * N;

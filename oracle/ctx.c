@@ -80,8 +80,33 @@ static uint64_t prod_except_mod(const uint64_t *mods, int lo, int hi, int skip, 
 static uint64_t *u64a(size_t n) { return (uint64_t *)calloc(n ? n : 1, sizeof(uint64_t)); }
 static double *f64a(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
+static or_ctx *ctx_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, int pn, const uint64_t *t,
+                          int tn, int gamma, uint64_t plain, const uint64_t *r_given);
+
 or_ctx *or_ctx_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, int pn,
                       const uint64_t *t, int tn, int gamma, uint64_t plain) {
+    return ctx_create(log_n, q, qn, p, pn, t, tn, gamma, plain, NULL);
+}
+
+/* The context of ciphertexts at `level` of `full` (the engine's level view):
+ * Q_l = q_0..q_level, the HPS basis R_l = the level + 2 moduli that follow
+ * q_level in the full context's [Q | R] table (the full context's own moduli,
+ * Q_l R_l one contiguous range of it), every Q_l-dependent table recomputed,
+ * and the relinearisation key of the full level read with its full shape. */
+or_ctx *or_ctx_level(const or_ctx *full, int level) {
+    if (level < 0 || level >= full->qn) return NULL;
+    or_ctx *c = ctx_create(full->log_n, full->mod + full->oQ, level + 1, full->mod + full->oP, full->pn,
+                           full->mod + full->oT, full->tn, full->gamma, full->plain,
+                           full->mod + full->oQ + level + 1);
+    if (c) {
+        c->key_dnum = full->key_dnum;
+        c->key_blocks = full->key_blocks;
+    }
+    return c;
+}
+
+static or_ctx *ctx_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, int pn, const uint64_t *t,
+                          int tn, int gamma, uint64_t plain, const uint64_t *r_given) {
     or_ctx *c = (or_ctx *)calloc(1, sizeof(or_ctx));
     c->log_n = log_n;
     c->n = (uint64_t)1 << log_n;
@@ -92,6 +117,8 @@ or_ctx *or_ctx_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, i
     c->gamma = gamma;
     c->dnum = (qn + pn - 1) / pn;
     c->ri_blocks = (pn + qn + gamma - 1) / gamma;
+    c->key_dnum = c->dnum;
+    c->key_blocks = c->ri_blocks;
     c->plain = plain;
     c->nmod = pn + qn + c->rn + tn + 1;
     c->oP = 0;
@@ -108,7 +135,9 @@ or_ctx *or_ctx_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, i
         if (q[i] < qmin) qmin = q[i];
     }
     /* include/BFVcontext.cuh:57-58 */
-    if (or_primes_below(c->n, qmin, c->rn, c->mod + c->oR) != 0) {
+    if (r_given) {
+        memcpy(c->mod + c->oR, r_given, (size_t)c->rn * sizeof(uint64_t));
+    } else if (or_primes_below(c->n, qmin, c->rn, c->mod + c->oR) != 0) {
         fprintf(stderr, "oracle: not enough R primes\n");
         free(c);
         return NULL;

@@ -26,6 +26,10 @@ struct or_ctx {
     int qn, pn, tn, rn, gamma;
     int dnum;       /* digits: ceil(qn / pn) */
     int ri_blocks;  /* ceil((pn + qn) / gamma) */
+    /* shape of the relinearisation key read by the inner product,
+     * [2][key_dnum][key_blocks][tn][N]: the context's own dnum / ri_blocks, or
+     * those of the full level for a level context (or_ctx_level) */
+    int key_dnum, key_blocks;
     uint64_t plain; /* t */
     int batching;   /* plain has a 2N-th root of unity */
     int nmod;       /* pn + qn + rn + tn + 1 */

@@ -99,12 +99,12 @@ void or_modup_pq_to_t(const or_ctx *c, const uint64_t *x_pq, uint64_t *out_t) {
     }
 }
 
-/* cin_t: [dnum][tn][N] (NTT over T), key_t: [2][dnum][ri_blocks][tn][N] (NTT)
- * -> out_t: [2][ri_blocks][tn][N] */
+/* cin_t: [dnum][tn][N] (NTT over T), key_t: [2][key_dnum][key_blocks][tn][N]
+ * (NTT; its first dnum digits and ri_blocks blocks) -> out_t: [2][ri_blocks][tn][N] */
 void or_inner_product(const or_ctx *c, const uint64_t *cin_t, const uint64_t *key_t,
                       uint64_t *out_t) {
     size_t N = c->n;
-    int tn = c->tn, B = c->ri_blocks, D = c->dnum;
+    int tn = c->tn, B = c->ri_blocks, D = c->dnum, KD = c->key_dnum, KB = c->key_blocks;
     for (int h = 0; h < 2; h++)
         for (int b = 0; b < B; b++)
             for (int k = 0; k < tn; k++) {
@@ -113,7 +113,7 @@ void or_inner_product(const or_ctx *c, const uint64_t *cin_t, const uint64_t *ke
                     u128 acc = 0;
                     for (int i = 0; i < D; i++) {
                         uint64_t x = cin_t[((size_t)i * tn + k) * N + n];
-                        uint64_t y = key_t[((((size_t)h * D + i) * B + b) * tn + k) * N + n];
+                        uint64_t y = key_t[((((size_t)h * KD + i) * KB + b) * tn + k) * N + n];
                         acc += (u128)x * y;
                     }
                     out_t[(((size_t)h * B + b) * tn + k) * N + n] = reduce128(acc, t);

@@ -56,6 +56,8 @@ typedef struct or_ctx or_ctx;
 
 /* Moduli are stored in one table in the order [P | Q | R | T] followed by the
  * plain modulus t (index or_ctx_mod_count()-1). */
+/* the context of ciphertexts at `level` of a full context (its own, owned by the caller) */
+or_ctx *or_ctx_level(const or_ctx *full, int level);
 or_ctx *or_ctx_create(int log_n, const uint64_t *q, int qn, const uint64_t *p, int pn,
                       const uint64_t *t, int tn, int gamma, uint64_t plain);
 void or_ctx_destroy(or_ctx *c);
@@ -134,6 +136,9 @@ void or_keygen_swk(const or_ctx *c, const uint8_t seed[32], const uint64_t *sk_p
 void or_encrypt(const or_ctx *c, const uint8_t seed[32], uint64_t entry, const uint64_t *pk,
                 const uint64_t *m, uint64_t *ct);
 void or_decrypt(const or_ctx *c, const uint64_t *sk_pq, const uint64_t *ct, uint64_t *m);
+/* modulus switching from_level -> to_level (include/BFVScheme.cuh:469-487):
+ * in [2][from_level + 1][N], out [2][to_level + 1][N], NTT form */
+void or_mod_switch(const or_ctx *c, const uint64_t *in, int from_level, uint64_t *out, int to_level);
 void or_mult_relin(const or_ctx *c, const uint64_t *rlk_t, const uint64_t *ct1,
                    const uint64_t *ct2, uint64_t *out);
 void or_mult_tensor(const or_ctx *c, const uint64_t *ct1, const uint64_t *ct2, uint64_t *d012);

@@ -40,6 +40,8 @@ def _load(path=LIB_PATH):
         "or_ctx_create": (C.c_void_p, [C.c_int, u64p, C.c_int, u64p, C.c_int, u64p, C.c_int,
                                        C.c_int, C.c_uint64]),
         "or_ctx_destroy": (None, [C.c_void_p]),
+        "or_ctx_level": (C.c_void_p, [C.c_void_p, C.c_int]),
+        "or_mod_switch": (None, [C.c_void_p, u64p, C.c_int, u64p, C.c_int]),
         "or_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
         "or_ctx_mod_count": (C.c_int, [C.c_void_p]),
         "or_ctx_moduli": (None, [C.c_void_p, u64p]),
@@ -124,12 +126,16 @@ def u64(x):
 class Ctx:
     """Oracle context.  Moduli order [P | Q | R | T | t]."""
 
-    def __init__(self, log_n, q, p, t, gamma, plain, _lib=None):
-        """_lib: another build of the oracle (load_variant); the default build otherwise."""
+    def __init__(self, log_n, q, p, t, gamma, plain, _lib=None, _handle=None):
+        """_lib: another build of the oracle (load_variant); the default build otherwise.
+        _handle: an oracle context made elsewhere (Ctx.level), owned by this object."""
         self._lib = _lib if _lib is not None else lib
         self.q_in, self.p_in, self.t_in = u64(q), u64(p), u64(t)
-        self.h = self._lib.or_ctx_create(log_n, P(self.q_in), len(q), P(self.p_in), len(p),
-                                   P(self.t_in), len(t), gamma, plain)
+        if _handle is not None:
+            self.h = _handle
+        else:
+            self.h = self._lib.or_ctx_create(log_n, P(self.q_in), len(q), P(self.p_in), len(p),
+                                             P(self.t_in), len(t), gamma, plain)
         if not self.h:
             raise ValueError("oracle rejected the parameter set")
         info = (C.c_int * 16)()
@@ -147,6 +153,20 @@ class Ctx:
         if getattr(self, "h", None):
             self._lib.or_ctx_destroy(self.h)
             self.h = None
+
+    def level(self, level):
+        """The oracle context of ciphertexts at `level` (or_ctx_level)."""
+        h = self._lib.or_ctx_level(self.h, level)
+        if not h:
+            raise ValueError("bad level")
+        return Ctx(self.log_n, self.q_in[:level + 1], self.p_in, self.t_in, self.gamma, self.plain,
+                   _lib=self._lib, _handle=h)
+
+    def mod_switch(self, ct, from_level, to_level):
+        ct = u64(ct)
+        out = np.zeros((2, to_level + 1, self.n), dtype=np.uint64)
+        self._lib.or_mod_switch(self.h, P(ct), from_level, P(out), to_level)
+        return out
 
     def table(self, m, which):
         ptr = self._lib.or_ctx_table(self.h, m, which)

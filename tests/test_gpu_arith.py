@@ -49,3 +49,43 @@ def test_arith_matches_oracle(setup):
     ctx.decode(dec)
     want = (m2.astype(object) - m1.astype(object)) % ctx.plain
     assert np.array_equal(kb.to_host(dec).astype(object), want)
+
+
+def test_plain_mult_matches_oracle(setup):
+    """Plaintext x ciphertext (BFVScheme::multConstAndEqual(Ciphertext&,
+    Plaintext&), include/BFVScheme.cuh:226-245): the plaintext lifted into
+    every Q limb and taken to the NTT domain (encode_ntt), then a pointwise
+    product with both components.  Equal to the oracle's NTT of the lifted
+    plaintext times the ciphertext mod q_i, and the product decrypts to the
+    slot-wise product."""
+    kb, ctx, oc, sk, pk = setup
+    rng = np.random.default_rng(5)
+    batch = 2
+    m1 = rng.integers(0, ctx.plain, size=(batch, ctx.n), dtype=np.uint64)
+    m2 = rng.integers(0, ctx.plain, size=ctx.n, dtype=np.uint64)
+    d1, d2 = kb.to_device(m1), kb.to_device(m2[None].copy())
+    ctx.encode(d1)
+    ctx.encode(d2)
+    c1 = ctx.encrypt(bytes([6]) * 32, pk, d1)
+    e2 = kb.to_host(d2)[0]
+    lifted = np.broadcast_to(e2, (ctx.qn, ctx.n)).copy()
+    pt = kb.to_device(lifted)
+    ctx.ntt(pt, pt, +1, ctx.oQ, ctx.qn)
+    out = ctx.empty(*c1.shape)
+    for b in range(batch):
+        for h in range(2):
+            ctx.poly_op("mul", c1[b, h], pt, out[b, h], ctx.oQ, ctx.qn)
+    got = kb.to_host(out)
+    o_pt = oc.ntt_fwd(lifted, oc.oQ, oc.qn)
+    assert np.array_equal(kb.to_host(pt), o_pt)
+    q = [int(v) for v in oc.mod[oc.oQ:oc.oQ + oc.qn]]
+    h1 = kb.to_host(c1)
+    for b in range(batch):
+        for h in range(2):
+            for i in range(ctx.qn):
+                want = (h1[b, h, i].astype(object) * o_pt[i].astype(object)) % q[i]
+                assert np.array_equal(got[b, h, i].astype(object), want), (b, h, i)
+    dec = ctx.decrypt(sk, out)
+    ctx.decode(dec)
+    want = (m1.astype(object) * m2[None].astype(object)) % ctx.plain
+    assert np.array_equal(kb.to_host(dec).astype(object), want)

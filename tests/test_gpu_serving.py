@@ -88,3 +88,40 @@ def test_large_rings_mix_matches_alone():
     stats = serving.serve(big, reqs, keep=True)
     assert stats["requests"] == len(reqs)
     _check_alone(big, reqs)
+
+
+def test_admission_defers_over_budget_burst(rings):
+    """A burst against a budget that holds only a few requests' working memory
+    at a time: batches are cut down and deferred until running ones release
+    their memory, the reserved bytes never exceed the budget, and every result
+    still equals the same request alone."""
+    from kunpbfv_amd import serving
+    cap = max(rings[ln].batch_bytes(op, 3) for ln in rings for op in serving.OPS)
+    budget = serving.MemoryBudget(capacity=cap)
+    reqs = serving.poisson_requests(qps=1e6, duration_s=1e-4, log_ns=[12, 13], seed=6, pool=6)
+    assert len(reqs) >= 60
+    stats = serving.serve(rings, reqs, keep=True, budget=budget)
+    adm = stats["admission"]
+    assert stats["requests"] == len(reqs) and adm["rejected"] == 0
+    assert adm["halvings"] > 0 and adm["deferrals"] > 0, adm
+    assert 0 < adm["peak_reserved_bytes"] <= cap and budget.used == 0
+    _check_alone(rings, reqs)
+
+
+def test_admission_rejects_what_never_fits(rings):
+    """A budget smaller than one request's working memory for some operations:
+    those requests are rejected (with the reason), the rest complete and equal
+    their alone-runs."""
+    from kunpbfv_amd import serving
+    cap = max(rings[ln].batch_bytes("add", 2) for ln in rings)
+    budget = serving.MemoryBudget(capacity=cap)
+    reqs = serving.poisson_requests(qps=1e5, duration_s=4e-4, log_ns=[12, 13], seed=7, pool=6)
+    stats = serving.serve(rings, reqs, keep=True, budget=budget)
+    never = {(ln, op) for ln in rings for op in serving.OPS if rings[ln].batch_bytes(op, 1) > cap}
+    assert never and any(op == "add" for _ln, op in {(r.log_n, r.op) for r in reqs})
+    for r in reqs:
+        assert bool(r.rejected) == ((r.log_n, r.op) in never), (r.rid, r.op, r.rejected)
+    done = [r for r in reqs if not r.rejected]
+    assert stats["admission"]["rejected"] == len(reqs) - len(done) > 0
+    assert stats["requests"] == len(done) > 0
+    _check_alone(rings, done)

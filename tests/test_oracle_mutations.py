@@ -8,6 +8,8 @@ fail on the planted build.
   2  Q_j -> T and PQ -> T with the float estimate rounded, not truncated
   3  K > 1 modDown with an alpha * P correction the reference does not make
   4  scale-and-round with nu rounded instead of truncated
+  5  modulus switching with the centred residue of the dropped limb (rounding
+     instead of the reference's floor)
 """
 import os
 import subprocess
@@ -18,6 +20,7 @@ import oracle_lib as orc
 import params as Pm
 import test_oracle_hps as th
 import test_oracle_ks as tk
+import test_oracle_levels as tl
 
 SRC = ["nt.c", "ntt.c", "ctx.c", "hps.c", "ks.c", "xsalsa20.c", "bfv.c"]
 ODIR = os.path.join(orc.ROOT, "oracle")
@@ -27,7 +30,7 @@ ODIR = os.path.join(orc.ROOT, "oracle")
 def variants(tmp_path_factory):
     d = tmp_path_factory.mktemp("oracle_mut")
     procs = {}
-    for n in (1, 2, 3, 4):
+    for n in (1, 2, 3, 4, 5):
         out = str(d / f"liboracle_mut{n}.so")
         cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", f"-DOR_MUTATION={n}", "-shared",
                "-o", out] + [os.path.join(ODIR, f) for f in SRC] + ["-lm"]
@@ -55,6 +58,7 @@ def test_real_build_passes_every_check():
         tk.check_modup_to_t(oc, "qj", 1)
         tk.check_modup_to_t(oc, "pq", 2)
         tk.check_moddown(oc)
+    tl.check_mod_switch_floor(tl.ctx_k1_n4096(), 3, 0)
 
 
 @pytest.mark.parametrize("n", [1, 4])
@@ -73,3 +77,8 @@ def test_modup_pin_rejects_rounded_estimate(variants, which, mk):
 def test_moddown_pin_rejects_alpha_correction(variants):
     with pytest.raises(AssertionError):
         tk.check_moddown(ctx_ref14(variants[3]))
+
+
+def test_mod_switch_pin_rejects_rounding_switch(variants):
+    with pytest.raises(AssertionError):
+        tl.check_mod_switch_floor(tl.ctx_k1_n4096(variants[5]), 3, 0)
