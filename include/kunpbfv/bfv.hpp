@@ -188,7 +188,16 @@ class BFVScheme {
     Key_decomp *addSWKey_23(SecretKey &secretKey, uint64_tt *s2, hipStream_t stream = 0);
 
     // encryption of zero (include/BFVScheme.cuh:87-116); l must be the full level L
+    // (bring a ciphertext down with rescaleToAndEqual)
     void encryptZero(Ciphertext &cipher, int l, int slots, hipStream_t stream = 0);
+    // modulus switching (include/BFVScheme.cuh:450-487): drop the top limbs
+    // down to `level` (rescaleAndEqual: one limb), x_i <- (x_i - x_l) q_l^-1 mod
+    // q_i; the ciphertext keeps its buffer of L + 1 limbs per component and
+    // sets cipher.l.  Ciphertexts below the full level decrypt, add, subtract,
+    // negate and multiply (with the full-level relinearisation key) at their
+    // level; operands of two different levels meet at the lower one.
+    void rescaleToAndEqual(Ciphertext &cipher, int level);
+    void rescaleAndEqual(Ciphertext &cipher);
     void encryptMsg(Ciphertext &cipher, Plaintext &message, hipStream_t stream = 0);
     void decryptMsg(Plaintext &plain, SecretKey &secretKey, Ciphertext &cipher, hipStream_t stream = 0);
 
@@ -247,6 +256,11 @@ class BFVScheme {
     std::vector<Key_decomp *> autoKey_vec_23;  // [logN], by d of X -> X^(2^d + 1)
 
    private:
+    // level-l view of the context and the compact [2][l + 1][N] form of a
+    // ciphertext (switched down from its own level when that is higher)
+    const kbfv_context *level_view(int l) const;
+    void to_compact(Ciphertext &c, int l, uint64_tt *dst, void *ws, size_t ws_len);
+    void from_compact(const uint64_tt *src, int l, Ciphertext &c);
     void seed_next(uint8_t out[32]);
     Key_decomp *galois_key(SecretKey &sk, uint64_t g, hipStream_t stream);
     void apply_galois(Ciphertext &res, Ciphertext &cipher, Key_decomp *key, uint64_t g);

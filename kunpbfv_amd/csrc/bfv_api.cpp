@@ -452,13 +452,78 @@ void BFVScheme::compute_t_QInv_c0c1c2(uint64_tt *axbx1, uint64_tt *axa, uint64_t
     hip_ok(hipStreamSynchronize(context.stream), "compute_t_QInv_c0c1c2");
 }
 
+// ---------------------------------------------------------------- levels
+const kbfv_context *BFVScheme::level_view(int l) const {
+    const kbfv_context *v = nullptr;
+    ok(kbfv_context_level(context.handle, l, &v), "level");
+    return v;
+}
+
+// c (level c.l >= l, limbs 0..c.l of each component at stride (L+1) N) ->
+// dst [2][l + 1][N]; ws: scratch for the switch when c.l > l
+void BFVScheme::to_compact(Ciphertext &c, int l, uint64_tt *dst, void *w, size_t wlen) {
+    const size_t N = context.N, full = (size_t)(context.L + 1) * N;
+    const size_t have = (size_t)(c.l + 1) * N;
+    if (c.l == l) {
+        hip_ok(hipMemcpy2DAsync(dst, have * 8, c.cipher_device, full * 8, have * 8, 2, hipMemcpyDeviceToDevice,
+                                context.stream),
+               "compact");
+        return;
+    }
+    // the switch reads its own compact copy: [2][c.l + 1][N] then its workspace
+    uint64_tt *src = (uint64_tt *)w;
+    const size_t src_bytes = (2 * have * 8 + 255) & ~size_t(255);
+    hip_ok(hipMemcpy2DAsync(src, have * 8, c.cipher_device, full * 8, have * 8, 2, hipMemcpyDeviceToDevice,
+                            context.stream),
+           "compact");
+    ok(kbfv_mod_switch(context.handle, (uint64_t *)src, c.l, (uint64_t *)dst, l, 1, (uint8_t *)w + src_bytes,
+                       wlen - src_bytes, context.stream),
+       "mod switch");
+}
+
+void BFVScheme::from_compact(const uint64_tt *src, int l, Ciphertext &c) {
+    const size_t N = context.N, full = (size_t)(context.L + 1) * N, have = (size_t)(l + 1) * N;
+    hip_ok(hipMemcpy2DAsync(c.cipher_device, full * 8, src, have * 8, have * 8, 2, hipMemcpyDeviceToDevice,
+                            context.stream),
+           "expand");
+    c.l = l;
+}
+
+void BFVScheme::rescaleToAndEqual(Ciphertext &cipher, int level) {
+    if (level < 0 || level > cipher.l) throw std::invalid_argument("rescaleToAndEqual: level out of range");
+    if (level == cipher.l) return;
+    const size_t N = context.N, ct = 2 * (size_t)(cipher.l + 1) * N * 8;
+    const size_t msw = kbfv_mod_switch_workspace_bytes(context.handle, cipher.l, 1);
+    uint8_t *w = (uint8_t *)ensure_ws(ct + 256 + ct + 256 + msw);
+    uint64_tt *out = (uint64_tt *)w;
+    to_compact(cipher, level, out, w + ((ct + 255) & ~size_t(255)), ws_bytes - ((ct + 255) & ~size_t(255)));
+    from_compact(out, level, cipher);
+    hip_ok(hipStreamSynchronize(context.stream), "rescaleTo");
+}
+
+void BFVScheme::rescaleAndEqual(Ciphertext &cipher) {
+    if (cipher.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
+    rescaleToAndEqual(cipher, cipher.l - 1);
+}
+
 void BFVScheme::decryptMsg(Plaintext &plain, SecretKey &sk, Ciphertext &cipher, hipStream_t stream) {
     plain.l = cipher.l;
     plain.slots = cipher.slots;
-    void *w = ensure_ws(kbfv_decrypt_workspace_bytes(context.handle, 1));
-    ok(kbfv_decrypt(context.handle, (uint64_t *)sk.sx_device, (uint64_t *)cipher.cipher_device,
-                    (uint64_t *)plain.mx_device, 1, w, ws_bytes, stream),
-       "decrypt");
+    if (cipher.l < context.L) {
+        const int l = cipher.l;
+        const size_t ct = 2 * (size_t)(l + 1) * context.N * 8, dws = kbfv_decrypt_workspace_bytes(level_view(l), 1);
+        uint8_t *w = (uint8_t *)ensure_ws(((ct + 255) & ~size_t(255)) + dws);
+        uint64_tt *cc = (uint64_tt *)w;
+        to_compact(cipher, l, cc, nullptr, 0);
+        ok(kbfv_decrypt(level_view(l), (uint64_t *)sk.sx_device, (uint64_t *)cc, (uint64_t *)plain.mx_device, 1,
+                        w + ((ct + 255) & ~size_t(255)), dws, context.stream),
+           "decrypt");
+    } else {
+        void *w = ensure_ws(kbfv_decrypt_workspace_bytes(context.handle, 1));
+        ok(kbfv_decrypt(context.handle, (uint64_t *)sk.sx_device, (uint64_t *)cipher.cipher_device,
+                        (uint64_t *)plain.mx_device, 1, w, ws_bytes, stream),
+           "decrypt");
+    }
     // the result is replicated into every limb, as the reference's HPS decrypt does
     for (int i = 1; i <= plain.L; ++i)
         hip_ok(hipMemcpyAsync(plain.mx_device + (size_t)i * plain.N, plain.mx_device, sizeof(uint64_tt) * plain.N,
@@ -468,17 +533,45 @@ void BFVScheme::decryptMsg(Plaintext &plain, SecretKey &sk, Ciphertext &cipher, 
 }
 
 void BFVScheme::add(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
-    res.l = std::min(c1.l, c2.l);
-    ok(kbfv_add(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
-                (uint64_t *)res.cipher_device, 1, context.stream),
-       "add");
+    const int l = std::min(c1.l, c2.l);
+    if (l == context.L) {
+        res.l = l;
+        ok(kbfv_add(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
+                    (uint64_t *)res.cipher_device, 1, context.stream),
+           "add");
+        return;
+    }
+    const size_t ct = ((2 * (size_t)(context.L + 1) * context.N * 8) + 255) & ~size_t(255);
+    const size_t msw = kbfv_mod_switch_workspace_bytes(context.handle, context.L, 1) + ct;
+    uint8_t *w = (uint8_t *)ensure_ws(2 * ct + msw);
+    uint64_tt *a = (uint64_tt *)w, *b = (uint64_tt *)(w + ct);
+    to_compact(c1, l, a, w + 2 * ct, ws_bytes - 2 * ct);
+    to_compact(c2, l, b, w + 2 * ct, ws_bytes - 2 * ct);
+    ok(kbfv_add(level_view(l), (uint64_t *)a, (uint64_t *)b, (uint64_t *)a, 1, context.stream), "add");
+    from_compact(a, l, res);
+    hip_ok(hipStreamSynchronize(context.stream), "add");
 }
 void BFVScheme::addAndEqual(Ciphertext &c1, Ciphertext &c2) { add(c1, c1, c2); }
 void BFVScheme::sub(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
-    res.l = std::min(c1.l, c2.l);
-    ok(kbfv_poly_sub(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
-                     (uint64_t *)res.cipher_device, context.p_num, context.q_num, 2, context.stream),
+    const int l = std::min(c1.l, c2.l);
+    if (l == context.L) {
+        res.l = l;
+        ok(kbfv_poly_sub(context.handle, (uint64_t *)c1.cipher_device, (uint64_t *)c2.cipher_device,
+                         (uint64_t *)res.cipher_device, context.p_num, context.q_num, 2, context.stream),
+           "sub");
+        return;
+    }
+    const size_t ct = ((2 * (size_t)(context.L + 1) * context.N * 8) + 255) & ~size_t(255);
+    const size_t msw = kbfv_mod_switch_workspace_bytes(context.handle, context.L, 1) + ct;
+    uint8_t *w = (uint8_t *)ensure_ws(2 * ct + msw);
+    uint64_tt *a = (uint64_tt *)w, *b = (uint64_tt *)(w + ct);
+    to_compact(c1, l, a, w + 2 * ct, ws_bytes - 2 * ct);
+    to_compact(c2, l, b, w + 2 * ct, ws_bytes - 2 * ct);
+    ok(kbfv_poly_sub(level_view(l), (uint64_t *)a, (uint64_t *)b, (uint64_t *)a, context.p_num, l + 1, 2,
+                     context.stream),
        "sub");
+    from_compact(a, l, res);
+    hip_ok(hipStreamSynchronize(context.stream), "sub");
 }
 void BFVScheme::subAndEqual(Ciphertext &c1, Ciphertext &c2) { sub(c1, c1, c2); }
 
@@ -518,7 +611,25 @@ void BFVScheme::addConstAndEqual(Ciphertext &cipher, uint64_tt cnst) {
 void BFVScheme::mult_23(Ciphertext &res, Ciphertext &c1, Ciphertext &c2) {
     if (c1.l == 0 || c2.l == 0) throw std::invalid_argument("Ciphertexts are on level 0");
     if (!rlk_23) throw std::invalid_argument("rotKey_23 not exists");
-    res.l = std::min(c1.l, c2.l);
+    const int l = std::min(c1.l, c2.l);
+    if (l < context.L) {
+        // both operands at level l (the higher one switched down), HomMult on the level view
+        const kbfv_context *v = level_view(l);
+        const size_t ct = ((2 * (size_t)(context.L + 1) * context.N * 8) + 255) & ~size_t(255);
+        const size_t mws = kbfv_mult_workspace_bytes(v, 1);
+        const size_t msw = kbfv_mod_switch_workspace_bytes(context.handle, context.L, 1) + ct;
+        uint8_t *w = (uint8_t *)ensure_ws(2 * ct + std::max(mws, msw));
+        uint64_tt *a = (uint64_tt *)w, *b = (uint64_tt *)(w + ct);
+        to_compact(c1, l, a, w + 2 * ct, ws_bytes - 2 * ct);
+        to_compact(c2, l, b, w + 2 * ct, ws_bytes - 2 * ct);
+        ok(kbfv_mult_relin(v, (uint64_t *)rlk_23->cipher_device, (uint64_t *)a, (uint64_t *)b, (uint64_t *)a, 1,
+                           w + 2 * ct, ws_bytes - 2 * ct, context.stream),
+           "mult");
+        from_compact(a, l, res);
+        hip_ok(hipStreamSynchronize(context.stream), "mult");
+        return;
+    }
+    res.l = l;
     void *w = ensure_ws(kbfv_mult_workspace_bytes(context.handle, 1));
     ok(kbfv_mult_relin(context.handle, (uint64_t *)rlk_23->cipher_device, (uint64_t *)c1.cipher_device,
                        (uint64_t *)c2.cipher_device, (uint64_t *)res.cipher_device, 1, w, ws_bytes, context.stream),
@@ -578,6 +689,12 @@ void BFVScheme::multAndEqual_23_batch(std::vector<Ciphertext *> &c1, std::vector
         if (c1[i]->N != (int)context.N || c1[i]->L != context.L || c2[i]->N != (int)context.N ||
             c2[i]->L != context.L)
             throw std::invalid_argument("ciphertext shape differs from the context");
+    }
+    bool below = false;
+    for (int i = 0; i < B; ++i) below = below || c1[i]->l < context.L || c2[i]->l < context.L;
+    if (below) {  // pairs below the full level: one HomMult each, at its level
+        for (int i = 0; i < B; ++i) mult_23(*c1[i], *c1[i], *c2[i]);
+        return;
     }
     const size_t ct = (size_t)2 * context.N * (context.L + 1);
     auto contiguous = [&](std::vector<Ciphertext *> &v) {

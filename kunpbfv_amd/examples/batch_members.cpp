@@ -10,6 +10,9 @@
 //   addAutoKey_23 / automorphismAndAdd
 //                           decrypts to m(X) + m(X^(2^d+1)) slot by slot
 //   rotKey_vec_23           a vector indexed by the rotation amount
+//   rescaleToAndEqual / rescaleAndEqual
+//                           switched-down ciphertexts decrypt, add across levels
+//                           and multiply below the full level
 // Usage: batch_members [N]   (default 16384)
 #include <cstdio>
 #include <cstdlib>
@@ -190,6 +193,37 @@ int main(int argc, char *argv[]) {
         scheme.addLeftRotKey_23(sk, 3);
         int bad = scheme.rotKey_vec_23.size() != (size_t)N || !scheme.rotKey_vec_23[3] || scheme.rotKey_vec_23[1];
         report("rotKey_vec_23 layout", bad);
+    }
+
+    // ---- levels: rescaleToAndEqual, then decrypt, add and HomMult below the
+    // full level (operands of two levels meet at the lower one), rescaleAndEqual
+    if (L >= 2) {
+        std::vector<uint64_tt> m1(slots), m2(slots);
+        for (auto &v : m1) v = (uint64_tt)rand() % t;
+        for (auto &v : m2) v = (uint64_tt)rand() % t;
+        Ciphertext a(N, L, L, slots), b(N, L, L, slots), s(N, L, L, slots);
+        encrypt_vec(a, m1);
+        encrypt_vec(b, m2);
+        scheme.rescaleToAndEqual(a, L - 1);  // a at L - 1, b at L
+        int bad = a.l != L - 1;
+        auto g = decrypt_vec(a);
+        for (int i = 0; i < slots; ++i) bad += g[i] != m1[i];
+        report("rescaleToAndEqual + decrypt", bad);
+        scheme.add(s, a, b);
+        bad = s.l != L - 1;
+        g = decrypt_vec(s);
+        for (int i = 0; i < slots; ++i) bad += g[i] != (m1[i] + m2[i]) % t;
+        report("add across levels", bad);
+        scheme.multAndEqual_23(a, b);
+        bad = a.l != L - 1;
+        g = decrypt_vec(a);
+        for (int i = 0; i < slots; ++i) bad += g[i] != (m1[i] * m2[i]) % t;
+        report("multAndEqual_23 below the full level", bad);
+        scheme.rescaleAndEqual(a);
+        bad = a.l != L - 2;
+        g = decrypt_vec(a);
+        for (int i = 0; i < slots; ++i) bad += g[i] != (m1[i] * m2[i]) % t;
+        report("rescaleAndEqual after the product", bad);
     }
 
     printf("total_wrong=%d\n", total_wrong);

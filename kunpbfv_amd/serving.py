@@ -9,11 +9,12 @@ call, so batches grow by themselves while the GPU is busy and rings of
 different sizes run side by side.  Completion is timed with HIP events against
 a start event, so a request's latency is (device completion - arrival).
 
-Admission is by device memory.  Keys, RGSW ciphertexts and the operand pools
-stay resident; what a batch needs on top (its workspace, the staged operands
-and its output) is reserved from one budget shared by every ring -- by
-default the device memory still free once the rings are resident, less a
-reserve -- and held until the batch's completion event fires.  A batch that
+Admission is by device memory.  Keys, RGSW ciphertexts, the operand pools
+and each ring's workspace stay resident; what a batch needs (its workspace
+share, the staged operands and its output) is reserved from one budget shared
+by every ring -- by default the device memory still free once the rings are
+resident, less a reserve -- and held until the batch's completion event
+fires.  A batch that
 does not fit is halved until it does; if not even one request fits, the
 requests stay queued (deferred) until running batches release memory, and a
 request that could not fit into the whole budget is rejected.
@@ -127,6 +128,19 @@ class Ring:
         self.max_batch = int(max_batch)
         self.stream = torch.cuda.Stream()
         self.enc_calls = 0
+        # the ring's workspace: one grow-only buffer on its stream (calls on
+        # one stream run in order, so they share it), sized here for the
+        # largest batch so that no allocation happens while requests wait
+        self._ws = None
+        self._stream_ws(max(self.workspace_bytes(op, self.max_batch) for op in OPS))
+
+    def _stream_ws(self, nbytes):
+        torch = self.torch
+        if self._ws is None or self._ws.numel() < nbytes:
+            with torch.cuda.stream(self.stream):
+                self._ws = None
+                self._ws = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
+        return self._ws
 
     def workspace_bytes(self, op, batch):
         lib, h = kb.lib, self.ctx.h
@@ -151,14 +165,16 @@ class Ring:
 
     def run(self, op, reqs, stream=None, keep=False, hold=None):
         """One batched call for the requests (same op), on `stream` (default:
-        the ring's).  Returns the output batch.  The call's workspace is
-        allocated here; `hold`, if given, receives every tensor the call uses
-        so the caller can keep them alive until the call completes."""
+        the ring's, whose workspace it shares with the ring's other calls).
+        Returns the output batch; `hold`, if given, receives every tensor the
+        call uses so the caller can keep them alive until the call completes."""
         torch, ctx = self.torch, self.ctx
         st = stream or self.stream
+        wsb = self.workspace_bytes(op, len(reqs))
+        ws = self._stream_ws(wsb) if st is self.stream else None
         with torch.cuda.stream(st):
-            wsb = self.workspace_bytes(op, len(reqs))
-            ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device="cuda")
+            if ws is None:  # a call on another stream gets its own workspace
+                ws = torch.empty(max(wsb, 256), dtype=torch.uint8, device="cuda")
             ia = torch.tensor([r.a for r in reqs], dtype=torch.long, device="cuda")
             if op == "encrypt":
                 m = self.pt.index_select(0, ia)
@@ -227,18 +243,14 @@ def serve(rings, reqs, max_inflight=2, keep=False, budget=None):
     start_ev.record()
     t0 = time.perf_counter()
     queues = collections.defaultdict(collections.deque)
-    inflight = collections.defaultdict(collections.deque)  # log_n -> (event, bytes, tensors) per batch
+    inflight = collections.defaultdict(collections.deque)  # log_n -> (event, reserved bytes) per batch
     batches = collections.Counter()
     deferrals = halvings = rejected = 0
     nxt = 0
 
-    def retire(fl, block=False):
-        while fl and (block or fl[0][0].query()):
-            ev, nbytes, _held = fl.popleft()
-            if block:
-                ev.synchronize()
-            budget.release(nbytes)
-            block = False
+    def retire(fl):
+        while fl and fl[0][0].query():
+            budget.release(fl.popleft()[1])
 
     while nxt < len(reqs) or any(queues.values()):
         now = time.perf_counter() - t0
@@ -247,32 +259,36 @@ def serve(rings, reqs, max_inflight=2, keep=False, budget=None):
             queues[(r.log_n, r.op)].append(r)
             nxt += 1
         issued = False
-        for fl in inflight.values():
-            retire(fl)
         for (log_n, op), q in queues.items():
             if not q:
                 continue
             fl = inflight[log_n]
+            retire(fl)
             if len(fl) >= max_inflight:
                 continue
             ring = rings[log_n]
             b = min(len(q), ring.max_batch)
-            while b > 1 and not budget.fits(ring.batch_bytes(op, b)):
-                b = (b + 1) // 2
-                halvings += 1
             need = ring.batch_bytes(op, b)
+            while b > 1 and not budget.fits(need):
+                b = (b + 1) // 2
+                need = ring.batch_bytes(op, b)
+                halvings += 1
             if not budget.reserve(need):
                 if need > budget.capacity:  # cannot fit even with nothing else in flight
                     r = q.popleft()
                     r.rejected = f"needs {need} bytes, budget {budget.capacity}"
                     rejected += 1
                 else:
+                    for other in inflight.values():  # memory may have come free on another ring
+                        retire(other)
                     deferrals += 1
                 continue
             take = [q.popleft() for _ in range(b)]
-            held = []
-            ring.run(op, take, keep=keep, hold=held)
-            fl.append((take[0].done_ev, need, held))
+            # the call's tensors live on the ring's stream: the allocator
+            # reuses their memory only for later work on that stream, so the
+            # reservation (not the tensors) is what is held until completion
+            ring.run(op, take, keep=keep)
+            fl.append((take[0].done_ev, need))
             batches[(log_n, op)] += 1
             issued = True
         if not issued:

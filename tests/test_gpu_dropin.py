@@ -48,7 +48,9 @@ def test_batch_and_scheme_members_driver(n):
     """multAndEqual_23_batch equals mult_23 on every pair (two calls, the
     second smaller, no device allocation after the first), encryptZero,
     compute_t_QInv_c0c1c2 with hps_decrypt_scale_and_round, addSWKey_23,
-    addAutoKey_23 / automorphismAndAdd and the vector rotKey_vec_23"""
+    addAutoKey_23 / automorphismAndAdd, the vector rotKey_vec_23, and (N =
+    16384, L = 3) rescaleToAndEqual / rescaleAndEqual with decryption, addition
+    across levels and HomMult below the full level"""
     assert os.path.exists(MEMBERS), "example driver not built (run __graft_entry__.build())"
     r = subprocess.run([MEMBERS, str(n)], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
