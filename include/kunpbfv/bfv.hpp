@@ -101,7 +101,8 @@ class BFVContext {
     // Wire format (no counterpart in the reference, which has no serialization;
     // kunpbfv.h KBFV_WIRE_*): blobs that load only into a context with the same
     // parameters.  A Plaintext travels as its limb-0 coefficients mod t and is
-    // replicated into every limb on load (the encode() layout).  A Key_decomp is
+    // replicated into every limb on load (the encode() layout); an encode_ntt
+    // plaintext (residues mod q) is refused by save with std::runtime_error.  A Key_decomp is
     // a relinearisation key, or a Galois key when galois_elt != 0.
     std::vector<uint8_tt> save(const Ciphertext &c);
     std::vector<uint8_tt> save(const Plaintext &m);

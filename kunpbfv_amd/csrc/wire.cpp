@@ -103,6 +103,25 @@ extern "C" size_t kbfv_wire_bytes(const kbfv_context *c, int kind, int count, ui
     return WIRE_HEADER + (size_t)count * rows_of(s) * c->n * sizeof(uint64_t);
 }
 
+namespace {
+// every residue below the modulus of its limb (rows cycle through the limb dimension)
+int check_range(const kbfv_context *c, const Shape &s, const uint64_t *w, size_t words) {
+    const size_t n = (size_t)c->n;
+    const size_t rows = words / n;
+    for (size_t r = 0; r < rows; r++) {
+        const uint64_t q = s.plain ? c->plain : c->mod[s.limb_mod0 + (int)(r % (size_t)s.limb_dim)];
+        const uint64_t *row = w + r * n;
+        uint64_t bad = 0;
+        for (size_t i = 0; i < n; i++) bad |= (uint64_t)(row[i] >= q);
+        KB_REQUIRE(!bad, s.plain ? "wire payload residue out of range in row %zu (a plaintext travels as its "
+                                   "coefficients mod t, the encode() form, not the encode_ntt form)"
+                                 : "wire payload residue out of range in row %zu",
+                   r);
+    }
+    return KBFV_OK;
+}
+}  // namespace
+
 extern "C" int kbfv_wire_save(const kbfv_context *c, int kind, const uint64_t *dev, int count, uint64_t aux,
                               void *buf, size_t cap, void *stream) {
     KB_REQUIRE(c && dev && buf && count > 0, "bad wire-save arguments");
@@ -122,6 +141,8 @@ extern "C" int kbfv_wire_save(const kbfv_context *c, int kind, const uint64_t *d
         std::memcpy(tmp.data(), w, words * 8);
         w = tmp.data();
     }
+    // what load would refuse is refused here, before a blob exists
+    KB_TRY(check_range(c, s, w, words));
     put32(p, WIRE_MAGIC);
     const uint16_t ver = WIRE_VERSION, k16 = (uint16_t)kind;
     std::memcpy(p + 4, &ver, 2), std::memcpy(p + 6, &k16, 2);
@@ -172,16 +193,7 @@ extern "C" int kbfv_wire_load(const kbfv_context *c, const void *buf, size_t len
         w = tmp.data();
     }
     KB_REQUIRE(kbfv_wire_checksum(w, words) == get64(p + 56), "wire payload checksum mismatch");
-    // every residue below the modulus of its limb (rows cycle through the limb dimension)
-    const size_t n = (size_t)c->n;
-    const size_t rows = words / n;
-    for (size_t r = 0; r < rows; r++) {
-        const uint64_t q = s.plain ? c->plain : c->mod[s.limb_mod0 + (int)(r % (size_t)s.limb_dim)];
-        const uint64_t *row = w + r * n;
-        uint64_t bad = 0;
-        for (size_t i = 0; i < n; i++) bad |= (uint64_t)(row[i] >= q);
-        KB_REQUIRE(!bad, "wire payload residue out of range in row %zu", r);
-    }
+    KB_TRY(check_range(c, s, w, words));
     hipStream_t st = (hipStream_t)stream;
     KB_HIP(hipMemcpyAsync(dev, w, words * 8, hipMemcpyHostToDevice, st));
     KB_HIP(hipStreamSynchronize(st));  // the caller's buffer may go away on return

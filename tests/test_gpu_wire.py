@@ -91,3 +91,21 @@ def test_loader_rejections(setup):
     # the context still loads the untouched blob after every rejection
     ctx.wire_load(blob, kb.WIRE_CIPHERTEXT, out)
     assert np.array_equal(kb.to_host(out), kb.to_host(ct))
+
+
+def test_saver_rejects_out_of_range_payload(setup):
+    """What the loader would refuse is refused at save time: a plaintext in the
+    NTT domain (residues mod q, not coefficients mod t, the encode_ntt form)
+    and a ciphertext with a residue equal to its limb's modulus."""
+    kb, ctx, sk, pk, m, ct = setup
+    pt = kb.to_device(m[:1].copy())
+    ctx.encode(pt)
+    ctx.wire_save(kb.WIRE_PLAINTEXT, pt)  # the encode() form saves
+    lifted = kb.to_device(np.broadcast_to(kb.to_host(pt)[0], (ctx.qn, ctx.n)).copy())
+    ctx.ntt(lifted, lifted, +1, ctx.oQ, ctx.qn)  # limb 0 of the encode_ntt form
+    ntt_form = lifted[0:1].contiguous()
+    assert (kb.to_host(ntt_form) >= ctx.plain).any()
+    _reject(kb, lambda: ctx.wire_save(kb.WIRE_PLAINTEXT, ntt_form), "encode()")
+    bad = kb.to_host(ct).copy()
+    bad[0, 0, 1, 7] = ctx.mod[ctx.oQ + 1]
+    _reject(kb, lambda: ctx.wire_save(kb.WIRE_CIPHERTEXT, kb.to_device(bad)), "range")
